@@ -1,0 +1,181 @@
+"""Rule x scenario simulation throughput on one MI355X.
+
+A step evaluates the whole plain-method workload once: 1,024 ramped
+half-and-half rules (EXTRA-18 terminals, depth 2-6 levels) on one synthetic
+uniform instance (n = 100, m = 2) over 500 common-random-number scenarios
+(Gamma(shape 4, mean 10) service times), producing every per-(rule,
+scenario) reward and the per-rule means.  Scenarios, rules and the instance
+are resident in HBM before timing; each step is one launch of the
+simulation kernel (plus the per-rule mean kernel) on torch's current stream,
+timed with HIP events on that stream.
+
+Prints one JSON line (the contract in the repository README of the driver):
+value = rule x scenario simulations per second on the one GPU, with a
+`roofline` object (FP64 operations of the simulation, counted by the CPU
+oracle's counting build on a sample of pairs, per kernel-second, against the
+MI355X FP64 vector peak) and a `cpu_baseline` object (the reference's own
+simulator, compiled from its sources, timed on the host cores on a bounded
+sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "solver-informed-dispatching-rules_amd"))
+
+import numpy as np  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 (vector and matrix) peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def need_rows(pk, mask):
+    """Per-rule slots the evaluator computes (the library's pruning rule:
+    only slots the rule reads and the mask makes live, plus dependencies)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import need_rows_for
+    return need_rows_for(pk, mask)
+
+
+def op_count_per_sim(wl, n_rules=24, n_scen=6):
+    """Average FP64 operations per (rule, scenario) simulation."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    from sigp.rules import pack_trees
+    O = Oracle(counting=True)
+    step = max(1, wl.packed.n_rules // n_rules)
+    trees = [wl.trees[i] for i in range(0, wl.packed.n_rules, step)][:n_rules]
+    pk = pack_trees(trees)
+    S = wl.sampled[:: max(1, wl.n_scen // n_scen)][:n_scen]
+    zp = O.zstats(wl.inst)
+    O.reset_ops()
+    O.evaluate(wl.inst, pk, S, S.shape[0], wl.mask, zp, need_rows=need_rows(pk, wl.mask), threads=1)
+    return O.ops() / (pk.n_rules * S.shape[0]), pk.n_rules * S.shape[0]
+
+
+def cpu_baseline(wl, zp, target_s=15.0):
+    """The reference's simulator (oracle/_ref) on host cores over a bounded
+    sample; falls back to the C restatement when the reference build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, Reference
+    from sigp.rules import pack_trees
+    threads = max(1, min(16, os.cpu_count() or 1))
+    if Reference.available():
+        impl, kind = Reference(), "reference"
+    else:
+        impl, kind = Oracle(), "port"
+
+    def run(nr, ns):
+        step = max(1, wl.packed.n_rules // nr)
+        trees = [wl.trees[i] for i in range(0, wl.packed.n_rules, step)][:nr]
+        pk = pack_trees(trees)
+        S = np.ascontiguousarray(wl.sampled[:ns])
+        t0 = time.perf_counter()
+        impl.evaluate(wl.inst, pk, S, ns, wl.mask, zp, threads=threads)
+        return time.perf_counter() - t0, pk.n_rules * ns
+
+    dt, sims = run(threads, 4)
+    rate = sims / max(dt, 1e-6)
+    want = int(rate * target_s)
+    nr = max(threads, min(wl.packed.n_rules, want // 32))
+    ns = max(1, min(wl.n_scen, want // nr))
+    dt, sims = run(nr, ns)
+    return {"value": sims / dt, "unit": "sims/s", "cores": threads, "kind": kind,
+            "sample": "%d rules (every %d-th) x first %d scenarios of the workload, %.1f s"
+                      % (nr, max(1, wl.packed.n_rules // nr), ns, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--rules", type=int, default=1024)
+    ap.add_argument("--scenarios", type=int, default=500)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(each GPU would evaluate its own share of rules); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from sigp.api import FactoryContext, device_info
+    from sigp.workloads import plain_small
+
+    t0 = time.perf_counter()
+    wl = plain_small(n_rules=args.rules, n_scen=args.scenarios)
+    log("workload built in %.1fs: %s" % (time.perf_counter() - t0, wl.name))
+    dev = torch.device("cuda:0")
+    name, n_cu, clk = device_info()
+    log("device", name, n_cu, "CUs")
+    ctx = FactoryContext(wl.inst)
+    P, T = wl.packed.n_rules, wl.n_scen
+    batch = ctx.batch(wl.packed, T, wl.mask)
+    sampled = torch.from_numpy(wl.sampled).to(dev)
+    pairs = torch.zeros(P * T, dtype=torch.float64, device=dev)
+    means = torch.zeros(P, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        batch.run(sampled.data_ptr(), T, pairs.data_ptr(), means.data_ptr(), stream.cuda_stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = [a.elapsed_time(b) for a, b in evs]
+    ms_step = float(np.mean(ms))
+    log("per-step ms:", ["%.2f" % v for v in ms], "wall %.3fs" % wall)
+    sims = P * T
+    value = sims / (ms_step / 1000.0)
+    # determinism / sanity of the timed results
+    pr = pairs.cpu().numpy()
+    assert np.all(np.isfinite(pr)) and np.all(pr == np.round(pr))
+
+    roof = None
+    if not args.no_roofline:
+        ops_per_sim, sample = op_count_per_sim(wl)
+        achieved = ops_per_sim * sims / (ms_step / 1000.0) / 1e12
+        roof = {"bound": "valu-fp64", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "ops_per_sim": round(ops_per_sim, 1), "ops_sample_sims": sample}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(wl, ctx.zp)
+        log("cpu baseline", cpu)
+    out = {
+        "metric": "rule x scenario sims/sec", "value": round(value, 1), "unit": "sims/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain method, small: 1 uniform instance n=100 m=2, %d rules "
+                               "(ramped half-and-half, depth 2-6 levels, EXTRA-18 terminals) x %d "
+                               "Gamma(4, mean 10) scenarios" % (P, T),
+                   "rules": P, "scenarios": T, "n": wl.inst.n, "m": wl.inst.m,
+                   "parallelism": "1 GPU, wave = rule x 64 scenarios"},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+/*
+ * sigp.h -- C ABI of the MI355X (gfx950) rule x scenario evaluator for SI-GP
+ * dispatching rules.  Drop-in for the reference's CUDA boundary
+ * (engine/factory.cuh, struct FactoryContext) and its CPU runner
+ * (engine/sim_host.hpp: sim_batch, zp_of): the same arrays go in, the same
+ * per-(rule, scenario) rewards come out, bit-identical.
+ *
+ * Plain C: pointers and sizes only.  Every call returns a sigp_status; the
+ * last error text of the calling thread is available from sigp_last_error().
+ * Host pointers are caller-owned and only read during the call; the library
+ * owns every device buffer it allocates.  A sigp_instance may be used from
+ * one thread at a time (the reference is single-threaded too).
+ */
+#ifndef SIGP_H
+#define SIGP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SIGP_N_TERMS 61     /* terminal slots (engine/vocab.hpp:16) */
+#define SIGP_MAX_N 512      /* customers per instance (engine/vocab.hpp:17) */
+#define SIGP_MAX_STACK 32   /* postfix stack depth (engine/vocab.hpp:19) */
+
+typedef enum {
+    SIGP_OK = 0,
+    SIGP_INVALID_ARG = 1,   /* sizes out of range, malformed postfix code, bad slot */
+    SIGP_DEVICE_ERROR = 2,  /* a HIP call failed */
+    SIGP_OOM = 3,           /* device allocation failed */
+    SIGP_NO_DEVICE = 4      /* no gfx950 device visible */
+} sigp_status;
+
+/* One STOPTW instance as the reference's SimData holds it
+ * (engine/sim_host.hpp:12-40): customer i = POI id i+1, depot = POI 0.
+ * dist is the (n+1)x(n+1) row-major travel-time matrix, dist_to_depot[i] =
+ * dist[(i+1)*(n+1)].  zp are the 10 z-statistics (5 means, 5 stds) the Z
+ * terminals read (engine/sim_host.hpp:96-105); pass NULL to have the library
+ * compute them (sigp_zstats). */
+typedef struct {
+    int32_t n;
+    int32_t m;
+    double tmax;
+    const double *scores, *durations, *open_t, *close_t;
+    const double *dist;
+    const double *dist_to_depot;
+    const double *zp;
+} sigp_instance_desc;
+
+/* Packed postfix rules, the reference's PackedTrees (engine/factory.cuh:20-40):
+ * rule p is ops[offsets[p] .. offsets[p]+lengths[p]) with two args per node;
+ * ops: -1 terminal (args[2k] = slot), -2 constant (args[2k] = round(v*1000)),
+ * 0..5 = + - * / min max, 6 = rmod (engine/vocab.hpp:88-120). */
+typedef struct {
+    const int32_t *ops;
+    const int32_t *args;
+    const int32_t *offsets;
+    const int32_t *lengths;
+    int32_t n_rules;
+    int64_t n_nodes;
+} sigp_rules;
+
+typedef struct sigp_instance sigp_instance;
+typedef struct sigp_batch sigp_batch;
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *sigp_last_error(void);
+
+/* Library/device information: device name into buf, CU count and the FP64
+ * lane-op peak (ops/s) the roofline uses. */
+sigp_status sigp_device_info(char *name_buf, size_t buf_len, int32_t *n_cu, double *clock_ghz);
+
+/* Replaces FactoryContext::FactoryContext (engine/factory.cuh:129-137):
+ * uploads the instance once and builds its derived tables (transposed
+ * distances, score/distance ratios, the ATRISK pow(var,0.5) table). */
+sigp_status sigp_instance_create(const sigp_instance_desc *desc, sigp_instance **out);
+
+/* Replaces FactoryContext::~FactoryContext (engine/factory.cuh:138-141). */
+sigp_status sigp_instance_destroy(sigp_instance *inst);
+
+/* Replaces zp_of / zstats (engine/sim_host.hpp:96-105, engine/sim_core.hpp:277-312),
+ * computed on the device; zp_out receives 10 doubles. */
+sigp_status sigp_zstats(const sigp_instance_desc *desc, double *zp_out);
+
+/* Replaces FactoryContext::evaluate / evaluate_grouped / evaluate_grouped_masks
+ * (engine/factory.cuh:144-164, 167-189, 193-216) in one entry point:
+ *   rule p reads scenario rows (p / group_size) * T + s, s in [0, T), of
+ *   `sampled` (row length n; n_rows rows in total) and terminal-mask row
+ *   p / mask_group of `masks` (n_masks rows of SIGP_N_TERMS int64 0/1).
+ *   evaluate(pk, sampled, N, mask)            = group_size >= P, T = N, mask_group >= P
+ *   evaluate_grouped(pk, s, G, T, gs, mask)   = group_size = gs, mask_group >= P
+ *   evaluate_grouped_masks(pk, s, T, gs, mg, masks) as is.
+ * out_pairs (nullable) receives P*T rewards, out[p*T + s]; out_mean
+ * (nullable) receives the P per-rule means, summed over s in order and
+ * divided by T as the reference's callers do (engine/gp_engine.cu:76-81).
+ * All pointers are host pointers; the call is synchronous. */
+sigp_status sigp_evaluate(sigp_instance *inst, const sigp_rules *rules,
+                          const double *sampled, int64_t n_rows, int32_t T,
+                          int32_t group_size, const int64_t *masks, int32_t n_masks,
+                          int32_t mask_group, double *out_pairs, double *out_mean);
+
+/* Device-resident form of sigp_evaluate for callers that keep scenarios and
+ * results in HBM (no counterpart in the reference, which copies per call;
+ * engine/factory.cuh:147-162).  sigp_batch_create validates and uploads the
+ * rules and masks once; sigp_batch_run launches on `stream` (a hipStream_t,
+ * NULL = default stream) and returns without waiting.  sampled_dev, out_pairs_dev
+ * and out_mean_dev are device pointers (out_* nullable). */
+sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *rules, int32_t T,
+                              int32_t group_size, const int64_t *masks, int32_t n_masks,
+                              int32_t mask_group, sigp_batch **out);
+sigp_status sigp_batch_run(sigp_batch *batch, const double *sampled_dev, int64_t n_rows,
+                           double *out_pairs_dev, double *out_mean_dev, void *stream);
+sigp_status sigp_batch_destroy(sigp_batch *batch);
+
+/* Number of rules and (rule, scenario) pairs in a batch. */
+sigp_status sigp_batch_shape(const sigp_batch *batch, int32_t *n_rules, int64_t *n_pairs);
+
+/* Device self-test hooks for the glibc-compatible transcendentals the
+ * simulator uses (erf, erfc): out[k] = f(x[k]) computed on the device.
+ * which: 0 = erf, 1 = erfc, 2 = exp, 3 = sqrt, 4 = pow(x, y[k]) (rmod). */
+sigp_status sigp_device_math(int32_t which, const double *x, const double *y, int64_t count,
+                             double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

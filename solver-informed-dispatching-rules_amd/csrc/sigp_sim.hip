@@ -1,0 +1,934 @@
+// SI-GP rule x scenario evaluator for MI355X (gfx950): kernels and the C ABI
+// declared in include/sigp.h.
+//
+// Work layout.  One wavefront simulates one rule on 64 scenarios (lane =
+// scenario, common random numbers across rules).  The rule's postfix code,
+// the candidate index of every decision and the neighbour index of every
+// terminal scan are wave-uniform, so instance data is read through the scalar
+// cache and the interpreter stack is indexed with a uniform stack pointer
+// (s_set_gpr_idx).  Lanes hold their own route state: current POI, clock,
+// reward and the visited set as NW 32-bit words.  A decision first builds,
+// per lane, the bitmask of customers feasible from the current POI (fnow) and
+// of those reachable before their close time (cnow), then walks the
+// candidates in index order; a candidate no lane can take is skipped by the
+// whole wave.
+//
+// Exactness.  Every floating-point expression follows the reference's
+// operand order (engine/sim_core.hpp, engine/sim_terms.hpp); the file is
+// compiled with -ffp-contract=off.  erf/erfc are glibc-compatible
+// (sigp_libm.h); pow(var, 0.5) of ATRISK is tabulated per instance with the
+// host libm call the reference makes; rewards are sums of scores, reduced in
+// scenario order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sigp.h"
+#include "sigp_libm.h"
+#include "sigp_terms.h"
+
+using namespace sigp;
+
+// ---------------------------------------------------------------------------
+// error state
+static thread_local std::string g_err;
+static sigp_status fail(sigp_status st, const std::string &msg) {
+    g_err = msg;
+    return st;
+}
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(e_ == hipErrorOutOfMemory ? SIGP_OOM : SIGP_DEVICE_ERROR,       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+extern "C" const char *sigp_last_error(void) { return g_err.c_str(); }
+
+// Internal bits of a rule's compute set (slots use bits 0..60).
+constexpr uint64_t kNeedNs = 1ull << 61;       // ns_val (engine/sim_core.hpp:100-102)
+constexpr uint64_t kNeedStruct = 1ull << 62;   // struct_feats_one (:116-121)
+constexpr uint64_t kNeedPre = 1ull << 63;      // VFRAC/NFEAS pre-pass (:57-71)
+constexpr uint64_t B(int s) { return 1ull << s; }
+
+// ---------------------------------------------------------------------------
+// instance setup kernels
+
+// distT = dist^T, rk[a][q] = score[q] / max(dist[a][q+1], 1e-10) and its
+// transpose (engine/sim_terms.hpp:31,89,109,427-428,558: the same divisions).
+__global__ void prep_tables_kernel(int n, const double *__restrict__ dist,
+                                   const double *__restrict__ score, double *__restrict__ distT,
+                                   double *__restrict__ rk, double *__restrict__ rkT) {
+    const int ld = n + 1;
+    const long long total = (long long)ld * ld;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(idx / ld), b = (int)(idx % ld);
+        distT[(size_t)b * ld + a] = dist[idx];
+        if (b >= 1) {
+            const int q = b - 1;
+            const double d = dist[idx];
+            const double v = score[q] / (d > 1e-10 ? d : 1e-10);
+            rk[(size_t)a * n + q] = v;
+            rkT[(size_t)q * ld + a] = v;
+        }
+    }
+}
+
+// z-statistic features of every customer at the initial state
+// (engine/sim_core.hpp:277-304): one lane per customer.
+template <int NW>
+__global__ void zfeat_kernel(DevInst I, double *__restrict__ feat, int *__restrict__ ok) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t vis[NW], fnow[NW], cnow[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) { vis[w] = 0; fnow[w] = 0; cnow[w] = 0; }
+    for (int j = 0; j < I.n; ++j) {   // feasibility from the depot at time 0
+        const double a = 0.0 + I.dist[j + 1];
+        if (a > I.close[j]) continue;
+        wset<NW>(cnow, j);
+        const double st = a > I.open[j] ? a : I.open[j];
+        if (st + I.dur[j] + I.d2d[j] > I.tmax) continue;
+        wset<NW>(fnow, j);
+    }
+    if (i >= I.n) return;
+    const double ta = I.dist[i + 1];
+    const double arr = ta;
+    int good = 1;
+    if (arr > I.close[i]) good = 0;
+    const double st = arr > I.open[i] ? arr : I.open[i];
+    if (good && st + I.dur[i] + I.d2d[i] > I.tmax) good = 0;
+    ok[i] = good;
+    if (!good) return;
+    const double to = I.open[i];
+    const double tsv = to > ta ? to : ta;
+    const double fin = tsv + I.dur[i];
+    const double ac = 0.0 + ta;
+    const double sc = ac > I.open[i] ? ac : I.open[i];
+    feat[(size_t)i * 5 + 0] = I.close[i];
+    feat[(size_t)i * 5 + 1] = t_reach2x<NW>(I, i, fin, 0.0, vis, false);
+    feat[(size_t)i * 5 + 2] = t_regret<NW>(I, i, fin, vis, fnow);
+    feat[(size_t)i * 5 + 3] = t_eloss<NW>(I, i, sc + I.dur[i], vis, cnow);
+    feat[(size_t)i * 5 + 4] = I.d2d[i];
+}
+
+// Sequential accumulation in customer order (engine/sim_core.hpp:298-311).
+__global__ void zreduce_kernel(int n, const double *__restrict__ feat, const int *__restrict__ ok,
+                               double *__restrict__ zp) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double zs[5] = {0, 0, 0, 0, 0}, zq[5] = {0, 0, 0, 0, 0};
+    double cnt = 0.0;
+    for (int i = 0; i < n; ++i) {
+        if (!ok[i]) continue;
+        for (int k = 0; k < 5; ++k) {
+            const double f = feat[(size_t)i * 5 + k];
+            zs[k] += f;
+            zq[k] += f * f;
+        }
+        cnt += 1.0;
+    }
+    if (cnt < 1.0) cnt = 1.0;
+    for (int k = 0; k < 5; ++k) {
+        const double mean = zs[k] / cnt;
+        double var = zq[k] / cnt - mean * mean;
+        if (var < 0.0) var = 0.0;
+        zp[k] = mean;
+        zp[5 + k] = sqrt(var) + 1e-9;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// simulation kernel
+
+struct RuleSet {
+    cip ops, args, off, len;
+    cup comp;    // per rule: slots to compute + internal bits 61..63
+    cup mbits;   // per rule: its terminal-mask row as bits
+    int P;
+};
+
+struct ScenSpec {
+    const double *sampled;   // rows of n durations
+    int T;                   // scenarios per rule
+    int group_size;          // rule p reads rows (p / group_size) * T + s
+    int nblk;                // ceil(T / 64)
+};
+
+// Postfix interpreter (engine/sim_core.hpp:11-41) with a uniform stack pointer.
+template <int STK>
+__device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L, const double *T) {
+    double stk[STK];
+    int sp = 0;
+    for (int k = 0; k < L; ++k) {
+        const int o = ops[k];
+        if (o == -1) {
+            stk[sp++] = T[args[2 * k]];
+        } else if (o == -2) {
+            stk[sp++] = args[2 * k] / 1000.0;
+        } else {
+            const double b = stk[sp - 1];
+            const double a = stk[sp - 2];
+            sp -= 2;
+            double r;
+            if (o == 0) r = a + b;
+            else if (o == 1) r = a - b;
+            else if (o == 2) r = a * b;
+            else if (o == 3) r = fabs(b) < 1e-10 ? 1.0 : a / b;
+            else if (o == 4) r = a < b ? a : b;
+            else if (o == 5) r = a > b ? a : b;
+            else {
+                const double base = a > 0.0 ? a : 0.0;
+                double e = b;
+                if (e < 0.0) e = 0.0;
+                else if (e > 1.0) e = 1.0;
+                r = sigp_pow_dd(base, 1.0 + e);
+            }
+            stk[sp++] = r;
+        }
+    }
+    return stk[0];
+}
+
+// One decision of pick_best (engine/sim_core.hpp:43-204) for every running lane.
+template <int NW, int STK>
+__device__ int decide(const DevInst &I, const cip ops, const cip args, int L, uint64_t comp,
+                      uint64_t mk, int cur, double ct, int rem, const uint32_t (&vis)[NW]) {
+    uint32_t fnow[NW], cnow[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) { fnow[w] = 0; cnow[w] = 0; }
+    double nvis = 0.0, nfeas = 0.0, fsc = 0.0, nfr = 0.0;
+    const bool pre = (comp & kNeedPre) != 0;
+    {
+        uint32_t vw = 0, fw = 0, cw = 0;
+        for (int j = 0; j < I.n; ++j) {
+            const int bit = j & 31;
+            if (bit == 0) { vw = wsel<NW>(vis, j >> 5); fw = 0; cw = 0; }
+            const bool visited = (vw >> bit) & 1u;
+            const double ta = I.distT[(size_t)(j + 1) * I.ld + cur];
+            const double a = ct + ta;
+            if (!(a > I.close[j])) {
+                cw |= 1u << bit;
+                const double st = a > I.open[j] ? a : I.open[j];
+                if (!(st + I.dur[j] + I.d2d[j] > I.tmax)) {
+                    fw |= 1u << bit;
+                    if (pre && !visited) {
+                        nfeas += 1.0;
+                        fsc += I.score[j];
+                        if (st + I.durrob[j] + I.d2d[j] <= I.tmax) nfr += 1.0;
+                    }
+                }
+            }
+            if (pre && visited) nvis += 1.0;
+            if (bit == 31 || j == I.n - 1) {
+                const int w = j >> 5;
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+                    fnow[k] = (w == k) ? fw : fnow[k];
+                    cnow[k] = (w == k) ? cw : cnow[k];
+                }
+            }
+        }
+    }
+    const double vfrac = pre ? nvis / I.n : 0.0;
+    const double sat = I.tmax > 0.0 ? ct / I.tmax : 0.0;
+
+    int best = -1;
+    double best_pri = -1e300;
+    double T[SIGP_N_TERMS];
+    double sbuf[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < I.n; ++i) {
+        const uint32_t b = 1u << (i & 31);
+        const int w = i >> 5;
+        const bool feas = !(wsel<NW>(vis, w) & b) && (wsel<NW>(fnow, w) & b);
+        if (__builtin_amdgcn_ballot_w64(feas) == 0) continue;
+        if (!feas) continue;
+        const double ta = I.distT[(size_t)(i + 1) * I.ld + cur];
+        const double oi = I.open[i], di = I.dur[i];
+        const double to = oi - ct;
+        const double tc = I.close[i] - ct;
+        const double tsv = to > ta ? to : ta;
+        const double tfv = tsv + di;
+        T[0] = I.score[i];
+        T[1] = di;
+        T[2] = to;
+        T[3] = tc;
+        T[4] = ta;
+        T[5] = I.d2d[i];
+        T[6] = tsv;
+        T[7] = tfv;
+        T[8] = tc - ta;
+        T[9] = rem * I.tmax + (I.tmax - ct);
+        const double fin = ct + tfv;
+        const double arr = ct + ta;
+        const double st_c = arr > oi ? arr : oi;
+        const double fin_c = st_c + di;
+
+        double nsv = 0.0;
+        if (comp & kNeedNs) nsv = t_ns<NW>(I, i, fin, vis);
+        if (comp & B(10)) T[10] = nsv;
+        if (comp & B(11)) T[11] = t_regret<NW>(I, i, fin, vis, fnow);
+        if (comp & B(12)) T[12] = t_ns_rob_m<NW>(I, i, fin, di, vis, 1.0, false);
+        if (comp & B(13)) T[13] = t_maxn<NW>(I, i, fin, vis);
+        if (comp & B(14)) T[14] = t_reach2x<NW>(I, i, fin, 0.0, vis, false);
+        if (comp & B(15)) T[15] = t_nstw<NW>(I, i, fin, vis);
+        if (comp & B(16)) T[16] = t_rand(i);
+        if (comp & kNeedStruct) struct_feats<NW>(I, i, fin_c, vis, fnow, sbuf);
+        if (comp & B(17)) T[17] = sbuf[0];
+        if (comp & B(18)) T[18] = sbuf[1];
+        if (comp & B(19)) T[19] = sbuf[2];
+        if (comp & B(20)) T[20] = sbuf[3];
+        if (comp & B(21)) T[21] = t_reach_k<NW>(I, i, st_c, vis, 1, false);
+        if (comp & B(22)) T[22] = t_reach_k<NW>(I, i, st_c, vis, 3, false);
+        if (comp & B(23)) T[23] = t_reach_k<NW>(I, i, st_c, vis, 5, false);
+        if (comp & B(24)) T[24] = sat;
+        if (comp & B(25)) T[25] = vfrac;
+        if (comp & B(26)) T[26] = nfeas;
+        if (comp & B(27)) T[27] = (double)rem;
+        if (comp & B(28)) T[28] = sbuf[1] / (nsv + 1.0);
+        if (comp & B(29)) T[29] = sbuf[1] - nsv;
+        if (comp & B(30)) T[30] = t_eloss<NW>(I, i, fin_c, vis, cnow);
+        if (comp & B(31)) T[31] = (T[3] - I.zp[0]) / I.zp[5];
+        if (comp & B(32)) T[32] = (T[14] - I.zp[1]) / I.zp[6];
+        if (comp & B(33)) T[33] = (T[11] - I.zp[2]) / I.zp[7];
+        if (comp & B(34)) T[34] = (T[30] - I.zp[3]) / I.zp[8];
+        if (comp & B(35)) T[35] = (T[5] - I.zp[4]) / I.zp[9];
+        if (comp & B(36)) T[36] = 0.1;
+        if (comp & B(37)) T[37] = 0.2;
+        if (comp & B(38)) T[38] = 0.3;
+        if (comp & B(39)) T[39] = 0.4;
+        if (comp & B(40)) T[40] = 0.5;
+        if (comp & B(41)) T[41] = t_reach2x<NW>(I, i, fin, di, vis, true);
+        if (comp & B(42)) {
+            double den = I.tmax - ct;
+            if (den < 1.0) den = 1.0;
+            T[42] = (ta + di) / den;
+        }
+        if (comp & B(43))
+            T[43] = t_reach2x<NW>(I, i, fin, di, vis, true) - t_regret<NW>(I, i, fin, vis, fnow);
+        if (comp & B(44)) T[44] = t_skiploss<NW>(I, i, cur, ct, vis);
+        if (comp & B(45))
+            T[45] = t_ns<NW>(I, i, fin, vis) - t_ns_rob_m<NW>(I, i, fin, di, vis, 1.0, false);
+        if (comp & B(46)) {
+            double den = ta + I.d2d[i];
+            if (den < 1.0) den = 1.0;
+            T[46] = (I.score[i] + t_ns_rob_m<NW>(I, i, fin, di, vis, 1.0, false)) / den;
+        }
+        if (comp & B(47)) T[47] = t_reach_k<NW>(I, i, st_c, vis, 3, true);
+        if (comp & B(48)) T[48] = t_reach_k<NW>(I, i, st_c, vis, 5, true);
+        if (comp & B(49)) T[49] = t_ns_alpha<NW>(I, i, fin, vis, false);
+        if (comp & B(50)) T[50] = t_ns_alpha<NW>(I, i, fin, vis, true);
+        if (comp & B(51)) T[51] = fsc;
+        if (comp & B(52)) T[52] = nfr;
+        if (comp & B(53)) T[53] = t_ns_rob_m<NW>(I, i, fin, di, vis, 0.5, true);
+        if (comp & B(54)) T[54] = t_ns_rob_m<NW>(I, i, fin, di, vis, 2.0, true);
+        if (comp & B(55)) T[55] = t_nnfwd<NW>(I, i, fin, vis);
+        if (comp & B(56)) T[56] = t_nnk3<NW>(I, i, fin, vis);
+        if (comp & B(57)) T[57] = t_nnchain<NW>(I, i, st_c, vis, 3);
+        if (comp & B(58)) T[58] = I.distT[cur] + I.d2d[i] - ta;   // dist[cur][0]
+        if (comp & B(59)) T[59] = t_detour<NW>(I, i, cur, fin, ta, vis);
+        if (comp & B(60)) T[60] = t_scorerank<NW>(I, i, fin, vis);
+        (void)mk;
+
+        double pri = eval_rule<STK>(ops, args, L, T);
+        if (pri != pri) pri = -1e300;
+        if (pri > best_pri) { best_pri = pri; best = i; }
+    }
+    return best;
+}
+
+// One wave = one rule x 64 scenarios (engine/sim_core.hpp:207-238 per lane).
+template <int NW, int STK>
+__global__ __launch_bounds__(256) void sim_kernel(DevInst I, RuleSet R, ScenSpec S,
+                                                  double *__restrict__ out) {
+    const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const int lane = threadIdx.x & 63;
+    const int p = wave / S.nblk;
+    if (p >= R.P) return;
+    const int blk = wave - p * S.nblk;
+    const int s = blk * 64 + lane;
+    const bool active = s < S.T;
+    const int off = R.off[p], L = R.len[p];
+    const cip ops = R.ops + off;
+    const cip args = R.args + 2 * (size_t)off;
+    const uint64_t comp = R.comp[p];
+    const uint64_t mk = R.mbits[p];
+    const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
+    const double *__restrict__ samp = S.sampled + row * (size_t)I.n;
+
+    uint32_t vis[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) vis[w] = 0;
+    int cur = 0, veh = 0;
+    double ct = 0.0, total = 0.0;
+    bool running = active && I.m > 0;
+    while (__builtin_amdgcn_ballot_w64(running) != 0) {
+        if (running) {
+            const int rem = I.m - veh - 1;
+            const int best = decide<NW, STK>(I, ops, args, L, comp, mk, cur, ct, rem, vis);
+            if (best < 0) {
+                ++veh;
+                cur = 0;
+                ct = 0.0;
+                if (veh >= I.m) running = false;
+            } else {
+                const int pb = best + 1;
+                const double ta = I.dist[(size_t)cur * I.ld + pb];
+                const double a = ct + ta;
+                const double st = a > I.open[best] ? a : I.open[best];
+                ct = st + samp[best];
+                cur = pb;
+                total += I.score[best];
+                wset<NW>(vis, best);
+            }
+        }
+    }
+    if (active) out[(size_t)p * S.T + s] = total;
+}
+
+// Per-rule mean in scenario order, then / T (engine/gp_engine.cu:76-81).
+__global__ void mean_kernel(const double *__restrict__ pairs, int P, int T, double *__restrict__ mean) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    double acc = 0.0;
+    const double *row = pairs + (size_t)p * T;
+    for (int s = 0; s < T; ++s) acc += row[s];
+    mean[p] = acc / T;
+}
+
+// Device math self-test (sigp_device_math).
+__global__ void math_kernel(int which, const double *__restrict__ x, const double *__restrict__ y,
+                            long long count, double *__restrict__ out) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double v = x[k];
+    double r;
+    switch (which) {
+    case 0: r = sigp_erf(v); break;
+    case 1: r = sigp_erfc(v); break;
+    case 2: r = sigp_exp(v); break;
+    case 3: r = sqrt(v); break;
+    default: r = sigp_pow_dd(v, y[k]); break;
+    }
+    out[k] = r;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+struct sigp_instance {
+    int n = 0, m = 0, K = 0;
+    double tmax = 0.0;
+    void *dbuf = nullptr;           // one device allocation holding every table
+    DevInst dev{};
+    std::vector<double> zp;
+    double *scen_buf = nullptr;     // cached staging for sigp_evaluate
+    size_t scen_cap = 0;
+    double *out_buf = nullptr;
+    size_t out_cap = 0;
+    double *mean_buf = nullptr;
+    size_t mean_cap = 0;
+};
+
+struct sigp_batch {
+    sigp_instance *inst = nullptr;
+    int P = 0, T = 0, group_size = 1, stk = 16;
+    int64_t rows_needed = 0;
+    void *dbuf = nullptr;
+    RuleSet rs{};
+    double *pairs_tmp = nullptr;    // used when the caller wants means only
+};
+
+static sigp_status check_device(int *dev_out) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(SIGP_NO_DEVICE, "no HIP device visible");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(SIGP_NO_DEVICE, std::string("device is ") + prop.gcnArchName +
+                                        ", this library is built for gfx950 only");
+    if (dev_out) *dev_out = dev;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_device_info(char *name_buf, size_t buf_len, int32_t *n_cu,
+                                        double *clock_ghz) {
+    int dev = 0;
+    sigp_status st = check_device(&dev);
+    if (st != SIGP_OK) return st;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (name_buf && buf_len) std::snprintf(name_buf, buf_len, "%s (%s)", prop.name, prop.gcnArchName);
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (clock_ghz) *clock_ghz = prop.clockRate / 1e6;
+    return SIGP_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static sigp_status validate_desc(const sigp_instance_desc *d) {
+    if (!d) return fail(SIGP_INVALID_ARG, "desc is NULL");
+    if (d->n < 0 || d->n > SIGP_MAX_N)
+        return fail(SIGP_INVALID_ARG, "n must be in [0, " + std::to_string(SIGP_MAX_N) + "]");
+    if (d->m < 0) return fail(SIGP_INVALID_ARG, "m must be >= 0");
+    if (d->n > 0 && (!d->scores || !d->durations || !d->open_t || !d->close_t ||
+                     !d->dist_to_depot))
+        return fail(SIGP_INVALID_ARG, "customer arrays must be non-NULL");
+    if (!d->dist) return fail(SIGP_INVALID_ARG, "dist must be non-NULL");
+    return SIGP_OK;
+}
+
+template <int NW>
+static void launch_zstats(const DevInst &I, double *feat, int *ok, double *zp) {
+    const int threads = 64 * ((I.n + 63) / 64 > 0 ? (I.n + 63) / 64 : 1);
+    zfeat_kernel<NW><<<1, threads>>>(I, feat, ok);
+    zreduce_kernel<<<1, 64>>>(I.n, feat, ok, zp);
+}
+
+static int nw_bucket(int n) { return n <= 128 ? 4 : (n <= 288 ? 9 : 16); }
+
+// Builds the device tables of an instance into one allocation.
+static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I) {
+    const int n = d->n, ld = n + 1;
+    I->n = n;
+    I->m = d->m;
+    I->tmax = d->tmax;
+    // duration classes for the ATRISK pow(var, 0.5) table (engine/sim_terms.hpp:229-252)
+    std::vector<int32_t> cls(n);
+    std::vector<double> cval;
+    {
+        std::map<uint64_t, int> seen;
+        for (int i = 0; i < n; ++i) {
+            uint64_t u;
+            std::memcpy(&u, &d->durations[i], 8);
+            auto it = seen.find(u);
+            if (it == seen.end()) {
+                const int k = (int)cval.size();
+                seen.emplace(u, k);
+                cval.push_back(d->durations[i]);
+                cls[i] = k;
+            } else {
+                cls[i] = it->second;
+            }
+        }
+    }
+    const int K = std::max(1, (int)cval.size());
+    I->K = K;
+    std::vector<double> pw2((size_t)K * K, 0.0), pw1(K, 0.0);
+    const double SIG = 0.2;
+    for (int a = 0; a < (int)cval.size(); ++a) {
+        const double sa = SIG * cval[a];
+        for (int b = 0; b < (int)cval.size(); ++b) {
+            const double sb = SIG * cval[b];
+            pw2[(size_t)a * K + b] = std::pow(sa * sa + sb * sb, 0.5);
+        }
+        pw1[a] = std::pow(0.0 + sa * sa, 0.5);
+    }
+    std::vector<double> durrob(n), sdv(n);
+    for (int i = 0; i < n; ++i) {
+        durrob[i] = d->durations[i] * (1.0 + kSigma);
+        sdv[i] = SIG * d->durations[i];
+    }
+
+    const size_t nn = (size_t)ld * ld;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
+    const size_t o_dist = take(nn * 8), o_distT = take(nn * 8), o_rk = take((size_t)ld * n * 8 + 8),
+                 o_rkT = take((size_t)n * ld * 8 + 8), o_open = take(n * 8 + 8),
+                 o_close = take(n * 8 + 8), o_dur = take(n * 8 + 8), o_d2d = take(n * 8 + 8),
+                 o_score = take(n * 8 + 8), o_durrob = take(n * 8 + 8), o_sdv = take(n * 8 + 8),
+                 o_cls = take(n * 4 + 8), o_pw2 = take((size_t)K * K * 8), o_pw1 = take(K * 8),
+                 o_zp = take(10 * 8), o_feat = take((size_t)n * 5 * 8 + 8), o_ok = take(n * 4 + 8);
+    char *base = nullptr;
+    HIP_TRY(hipMalloc((void **)&base, off));
+    I->dbuf = base;
+    auto up = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    HIP_TRY(up(o_dist, d->dist, nn * 8));
+    HIP_TRY(up(o_open, d->open_t, n * 8));
+    HIP_TRY(up(o_close, d->close_t, n * 8));
+    HIP_TRY(up(o_dur, d->durations, n * 8));
+    HIP_TRY(up(o_d2d, d->dist_to_depot, n * 8));
+    HIP_TRY(up(o_score, d->scores, n * 8));
+    HIP_TRY(up(o_durrob, durrob.data(), n * 8));
+    HIP_TRY(up(o_sdv, sdv.data(), n * 8));
+    HIP_TRY(up(o_cls, cls.data(), n * 4));
+    HIP_TRY(up(o_pw2, pw2.data(), (size_t)K * K * 8));
+    HIP_TRY(up(o_pw1, pw1.data(), K * 8));
+
+    DevInst &D = I->dev;
+    D.n = n;
+    D.m = d->m;
+    D.ld = ld;
+    D.K = K;
+    D.tmax = d->tmax;
+    auto cd = [&](size_t o) { return (cdp)(base + o); };
+    D.dist = cd(o_dist);
+    D.distT = cd(o_distT);
+    D.rk = cd(o_rk);
+    D.rkT = cd(o_rkT);
+    D.open = cd(o_open);
+    D.close = cd(o_close);
+    D.dur = cd(o_dur);
+    D.d2d = cd(o_d2d);
+    D.score = cd(o_score);
+    D.durrob = cd(o_durrob);
+    D.sdv = cd(o_sdv);
+    D.dcls = (cip)(base + o_cls);
+    D.pw2 = cd(o_pw2);
+    D.pw1 = cd(o_pw1);
+    D.zp = cd(o_zp);
+
+    {
+        const long long total = (long long)nn;
+        int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+        if (blocks < 1) blocks = 1;
+        prep_tables_kernel<<<blocks, 256>>>(n, (const double *)(base + o_dist),
+                                            (const double *)(base + o_score), (double *)(base + o_distT),
+                                            (double *)(base + o_rk), (double *)(base + o_rkT));
+        HIP_TRY(hipGetLastError());
+    }
+    I->zp.assign(10, 0.0);
+    if (d->zp) {
+        std::memcpy(I->zp.data(), d->zp, 80);
+        HIP_TRY(up(o_zp, d->zp, 80));
+    } else {
+        double *feat = (double *)(base + o_feat);
+        int *okp = (int *)(base + o_ok);
+        double *zpd = (double *)(base + o_zp);
+        switch (nw_bucket(n)) {
+        case 4: launch_zstats<4>(D, feat, okp, zpd); break;
+        case 9: launch_zstats<9>(D, feat, okp, zpd); break;
+        default: launch_zstats<16>(D, feat, okp, zpd); break;
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpy(I->zp.data(), zpd, 80, hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_instance_create(const sigp_instance_desc *desc, sigp_instance **out) {
+    if (!out) return fail(SIGP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    sigp_status st = validate_desc(desc);
+    if (st != SIGP_OK) return st;
+    st = check_device(nullptr);
+    if (st != SIGP_OK) return st;
+    sigp_instance *I = new sigp_instance();
+    st = build_instance(desc, I);
+    if (st != SIGP_OK) {
+        if (I->dbuf) (void)hipFree(I->dbuf);
+        delete I;
+        return st;
+    }
+    *out = I;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_instance_destroy(sigp_instance *inst) {
+    if (!inst) return SIGP_OK;
+    if (inst->dbuf) (void)hipFree(inst->dbuf);
+    if (inst->scen_buf) (void)hipFree(inst->scen_buf);
+    if (inst->out_buf) (void)hipFree(inst->out_buf);
+    if (inst->mean_buf) (void)hipFree(inst->mean_buf);
+    delete inst;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_zstats(const sigp_instance_desc *desc, double *zp_out) {
+    if (!zp_out) return fail(SIGP_INVALID_ARG, "zp_out is NULL");
+    sigp_instance_desc d = *desc;
+    d.zp = nullptr;
+    sigp_instance *I = nullptr;
+    sigp_status st = sigp_instance_create(&d, &I);
+    if (st != SIGP_OK) return st;
+    std::memcpy(zp_out, I->zp.data(), 80);
+    return sigp_instance_destroy(I);
+}
+
+// Rule validation and the per-rule compute set.
+// A slot's value is live in the reference when (engine/sim_core.hpp:88-197):
+// slots 0-9 always; 11/14/30 when their own or their Z partner's mask bit is
+// set (33/32/34); any other slot when its mask bit is set.  A leaf whose slot
+// is not live reads 0.0 and is rewritten to the constant 0.
+static bool slot_live(int s, uint64_t mk) {
+    if (s <= 9) return true;
+    if (s == 11) return (mk & (B(11) | B(33))) != 0;
+    if (s == 14) return (mk & (B(14) | B(32))) != 0;
+    if (s == 30) return (mk & (B(30) | B(34))) != 0;
+    return (mk & B(s)) != 0;
+}
+
+static uint64_t compute_set(uint64_t used, uint64_t mk) {
+    uint64_t c = 0;
+    for (int s = 0; s < SIGP_N_TERMS; ++s)
+        if ((used & B(s)) && slot_live(s, mk)) c |= B(s);
+    if (c & B(32)) c |= B(14);
+    if (c & B(33)) c |= B(11);
+    if (c & B(34)) c |= B(30);
+    if (c & (B(10) | B(28) | B(29))) c |= kNeedNs;
+    if (c & (B(17) | B(18) | B(19) | B(20) | B(28) | B(29))) c |= kNeedStruct;
+    if (c & (B(25) | B(26) | B(51) | B(52))) c |= kNeedPre;
+    return c;
+}
+
+extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *rules, int32_t T,
+                                         int32_t group_size, const int64_t *masks, int32_t n_masks,
+                                         int32_t mask_group, sigp_batch **out) {
+    if (!out) return fail(SIGP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!inst || !rules) return fail(SIGP_INVALID_ARG, "inst and rules must be non-NULL");
+    const int P = rules->n_rules;
+    if (P < 0) return fail(SIGP_INVALID_ARG, "n_rules < 0");
+    if (T < 0) return fail(SIGP_INVALID_ARG, "T < 0");
+    if (group_size < 1 || mask_group < 1)
+        return fail(SIGP_INVALID_ARG, "group_size and mask_group must be >= 1");
+    if (P > 0 && (!rules->ops || !rules->args || !rules->offsets || !rules->lengths))
+        return fail(SIGP_INVALID_ARG, "rule arrays must be non-NULL");
+    if (P > 0 && (!masks || n_masks < 1)) return fail(SIGP_INVALID_ARG, "masks missing");
+    if ((long long)(P > 0 ? (P - 1) / mask_group : 0) >= (long long)std::max(n_masks, 1))
+        return fail(SIGP_INVALID_ARG, "mask row (n_rules-1)/mask_group out of range");
+
+    std::vector<int32_t> ops, args, offs(P), lens(P);
+    std::vector<uint64_t> comp(P), mb(P);
+    int max_depth = 1;
+    for (int p = 0; p < P; ++p) {
+        const int64_t o = rules->offsets[p], L = rules->lengths[p];
+        if (o < 0 || L < 1 || o + L > rules->n_nodes)
+            return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": offset/length out of range");
+        const int64_t *mrow = masks + (size_t)(p / mask_group) * SIGP_N_TERMS;
+        uint64_t mk = 0;
+        for (int s = 0; s < SIGP_N_TERMS; ++s)
+            if (mrow[s]) mk |= B(s);
+        uint64_t used = 0;
+        int depth = 0;
+        for (int64_t k = 0; k < L; ++k) {
+            const int op = rules->ops[o + k];
+            if (op == -1) {
+                const int s = rules->args[2 * (o + k)];
+                if (s < 0 || s >= SIGP_N_TERMS)
+                    return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": bad terminal slot");
+                used |= B(s);
+                ++depth;
+            } else if (op == -2) {
+                ++depth;
+            } else if (op >= 0 && op <= 6) {
+                if (depth < 2)
+                    return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack underflow");
+                --depth;
+            } else {
+                return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": bad opcode");
+            }
+            max_depth = std::max(max_depth, depth);
+        }
+        if (depth != 1)
+            return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": postfix does not reduce to one value");
+        if (max_depth > SIGP_MAX_STACK)
+            return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack deeper than " +
+                                              std::to_string(SIGP_MAX_STACK));
+        const uint64_t c = compute_set(used, mk);
+        offs[p] = (int32_t)ops.size();
+        lens[p] = (int32_t)L;
+        for (int64_t k = 0; k < L; ++k) {
+            int op = rules->ops[o + k];
+            int a0 = rules->args[2 * (o + k)], a1 = rules->args[2 * (o + k) + 1];
+            if (op == -1 && !(c & B(a0))) { op = -2; a0 = 0; a1 = 0; }   // dead slot reads 0.0
+            ops.push_back(op);
+            args.push_back(a0);
+            args.push_back(a1);
+        }
+        comp[p] = c;
+        mb[p] = mk;
+    }
+    sigp_batch *bt = new sigp_batch();
+    bt->inst = inst;
+    bt->P = P;
+    bt->T = T;
+    bt->group_size = group_size;
+    bt->stk = max_depth <= 16 ? 16 : 32;
+    bt->rows_needed = P > 0 ? (int64_t)((P - 1) / group_size + 1) * T : 0;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes + 8); return o; };
+    const size_t o_ops = take(ops.size() * 4), o_args = take(args.size() * 4), o_off = take(P * 4),
+                 o_len = take(P * 4), o_comp = take(P * 8), o_mb = take(P * 8);
+    char *base = nullptr;
+    hipError_t e = hipMalloc((void **)&base, off);
+    if (e != hipSuccess) { delete bt; return fail(SIGP_OOM, "batch allocation failed"); }
+    bt->dbuf = base;
+    auto up = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
+        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    if (up(o_ops, ops.data(), ops.size() * 4) != hipSuccess ||
+        up(o_args, args.data(), args.size() * 4) != hipSuccess ||
+        up(o_off, offs.data(), P * 4) != hipSuccess || up(o_len, lens.data(), P * 4) != hipSuccess ||
+        up(o_comp, comp.data(), P * 8) != hipSuccess || up(o_mb, mb.data(), P * 8) != hipSuccess) {
+        (void)hipFree(base);
+        delete bt;
+        return fail(SIGP_DEVICE_ERROR, "batch upload failed");
+    }
+    bt->rs.ops = (cip)(base + o_ops);
+    bt->rs.args = (cip)(base + o_args);
+    bt->rs.off = (cip)(base + o_off);
+    bt->rs.len = (cip)(base + o_len);
+    bt->rs.comp = (cup)(base + o_comp);
+    bt->rs.mbits = (cup)(base + o_mb);
+    bt->rs.P = P;
+    *out = bt;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_batch_destroy(sigp_batch *bt) {
+    if (!bt) return SIGP_OK;
+    if (bt->dbuf) (void)hipFree(bt->dbuf);
+    if (bt->pairs_tmp) (void)hipFree(bt->pairs_tmp);
+    delete bt;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_batch_shape(const sigp_batch *bt, int32_t *n_rules, int64_t *n_pairs) {
+    if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
+    if (n_rules) *n_rules = bt->P;
+    if (n_pairs) *n_pairs = (int64_t)bt->P * bt->T;
+    return SIGP_OK;
+}
+
+template <int NW, int STK>
+static void launch_sim(const sigp_batch *bt, const double *sampled, double *out, hipStream_t stream) {
+    const sigp_instance *I = bt->inst;
+    ScenSpec S;
+    S.sampled = sampled;
+    S.T = bt->T;
+    S.group_size = bt->group_size;
+    S.nblk = (bt->T + 63) / 64;
+    const long long waves = (long long)bt->P * S.nblk;
+    const unsigned grid = (unsigned)((waves + 3) / 4);
+    sim_kernel<NW, STK><<<grid, 256, 0, stream>>>(I->dev, bt->rs, S, out);
+}
+
+extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,
+                                      double *out_pairs_dev, double *out_mean_dev, void *stream) {
+    if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
+    if (bt->P == 0 || bt->T == 0) {
+        if (out_mean_dev && bt->P > 0 && bt->T == 0)
+            return fail(SIGP_INVALID_ARG, "mean over T = 0 scenarios");
+        return SIGP_OK;
+    }
+    if (!sampled_dev) return fail(SIGP_INVALID_ARG, "sampled_dev is NULL");
+    if (n_rows < bt->rows_needed)
+        return fail(SIGP_INVALID_ARG, "sampled has " + std::to_string(n_rows) + " rows, rules need " +
+                                          std::to_string(bt->rows_needed));
+    hipStream_t s = (hipStream_t)stream;
+    double *pairs = out_pairs_dev;
+    if (!pairs) {
+        if (!out_mean_dev) return SIGP_OK;
+        if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->P * bt->T * 8));
+        pairs = bt->pairs_tmp;
+    }
+    const int nwb = nw_bucket(bt->inst->n);
+    if (bt->stk == 16) {
+        if (nwb == 4) launch_sim<4, 16>(bt, sampled_dev, pairs, s);
+        else if (nwb == 9) launch_sim<9, 16>(bt, sampled_dev, pairs, s);
+        else launch_sim<16, 16>(bt, sampled_dev, pairs, s);
+    } else {
+        if (nwb == 4) launch_sim<4, 32>(bt, sampled_dev, pairs, s);
+        else if (nwb == 9) launch_sim<9, 32>(bt, sampled_dev, pairs, s);
+        else launch_sim<16, 32>(bt, sampled_dev, pairs, s);
+    }
+    HIP_TRY(hipGetLastError());
+    if (out_mean_dev) {
+        mean_kernel<<<(bt->P + 255) / 256, 256, 0, s>>>(pairs, bt->P, bt->T, out_mean_dev);
+        HIP_TRY(hipGetLastError());
+    }
+    return SIGP_OK;
+}
+
+static sigp_status ensure(double **buf, size_t *cap, size_t bytes) {
+    if (*cap >= bytes && *buf) return SIGP_OK;
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    HIP_TRY(hipMalloc((void **)buf, bytes ? bytes : 8));
+    *cap = bytes;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_evaluate(sigp_instance *inst, const sigp_rules *rules, const double *sampled,
+                                     int64_t n_rows, int32_t T, int32_t group_size, const int64_t *masks,
+                                     int32_t n_masks, int32_t mask_group, double *out_pairs,
+                                     double *out_mean) {
+    if (!inst) return fail(SIGP_INVALID_ARG, "inst is NULL");
+    sigp_batch *bt = nullptr;
+    sigp_status st = sigp_batch_create(inst, rules, T, group_size, masks, n_masks, mask_group, &bt);
+    if (st != SIGP_OK) return st;
+    const int P = bt->P;
+    if (P == 0 || T == 0) {
+        sigp_batch_destroy(bt);
+        if (out_mean && P > 0) return fail(SIGP_INVALID_ARG, "mean over T = 0 scenarios");
+        return SIGP_OK;
+    }
+    if (!sampled) { sigp_batch_destroy(bt); return fail(SIGP_INVALID_ARG, "sampled is NULL"); }
+    if (n_rows < bt->rows_needed) {
+        sigp_batch_destroy(bt);
+        return fail(SIGP_INVALID_ARG, "sampled has too few rows");
+    }
+    const size_t sbytes = (size_t)n_rows * inst->n * 8, obytes = (size_t)P * T * 8;
+    if ((st = ensure(&inst->scen_buf, &inst->scen_cap, sbytes)) != SIGP_OK ||
+        (st = ensure(&inst->out_buf, &inst->out_cap, obytes)) != SIGP_OK ||
+        (st = ensure(&inst->mean_buf, &inst->mean_cap, (size_t)P * 8)) != SIGP_OK) {
+        sigp_batch_destroy(bt);
+        return st;
+    }
+    hipError_t e = sbytes ? hipMemcpy(inst->scen_buf, sampled, sbytes, hipMemcpyHostToDevice) : hipSuccess;
+    if (e != hipSuccess) { sigp_batch_destroy(bt); return fail(SIGP_DEVICE_ERROR, "scenario upload failed"); }
+    st = sigp_batch_run(bt, inst->scen_buf, n_rows, inst->out_buf, out_mean ? inst->mean_buf : nullptr, nullptr);
+    if (st == SIGP_OK) {
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) st = fail(SIGP_DEVICE_ERROR, std::string("kernel failed: ") + hipGetErrorString(e));
+    }
+    if (st == SIGP_OK && out_pairs) {
+        e = hipMemcpy(out_pairs, inst->out_buf, obytes, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = fail(SIGP_DEVICE_ERROR, "result download failed");
+    }
+    if (st == SIGP_OK && out_mean) {
+        e = hipMemcpy(out_mean, inst->mean_buf, (size_t)P * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) st = fail(SIGP_DEVICE_ERROR, "mean download failed");
+    }
+    sigp_batch_destroy(bt);
+    return st;
+}
+
+extern "C" sigp_status sigp_device_math(int32_t which, const double *x, const double *y, int64_t count,
+                                        double *out) {
+    if (count < 0 || (count > 0 && (!x || !out)) || (which == 4 && count > 0 && !y))
+        return fail(SIGP_INVALID_ARG, "bad arguments");
+    if (count == 0) return SIGP_OK;
+    sigp_status st = check_device(nullptr);
+    if (st != SIGP_OK) return st;
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    const size_t bytes = (size_t)count * 8;
+    HIP_TRY(hipMalloc((void **)&dx, bytes));
+    HIP_TRY(hipMalloc((void **)&dy, bytes));
+    HIP_TRY(hipMalloc((void **)&dout, bytes));
+    HIP_TRY(hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice));
+    if (y) HIP_TRY(hipMemcpy(dy, y, bytes, hipMemcpyHostToDevice));
+    else HIP_TRY(hipMemset(dy, 0, bytes));
+    math_kernel<<<(unsigned)((count + 255) / 256), 256>>>(which, dx, dy, count, dout);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost));
+    (void)hipFree(dx);
+    (void)hipFree(dy);
+    (void)hipFree(dout);
+    return SIGP_OK;
+}

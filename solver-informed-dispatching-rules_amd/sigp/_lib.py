@@ -1,0 +1,79 @@
+"""ctypes binding of include/sigp.h (libsigp.so, built in-tree by sigp.build).
+
+There is no fallback: if the library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "EXPORTED"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libsigp.so")
+
+# Every symbol include/sigp.h declares.
+EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp_instance_destroy",
+            "sigp_zstats", "sigp_evaluate", "sigp_batch_create", "sigp_batch_run",
+            "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math"]
+
+_STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
+           4: "SIGP_NO_DEVICE"}
+
+
+class SigpError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__("%s: %s" % (_STATUS.get(status, str(status)), msg))
+        self.status = status
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("m", C.c_int32), ("tmax", C.c_double),
+                ("scores", C.c_void_p), ("durations", C.c_void_p), ("open_t", C.c_void_p),
+                ("close_t", C.c_void_p), ("dist", C.c_void_p), ("dist_to_depot", C.c_void_p),
+                ("zp", C.c_void_p)]
+
+
+class Rules(C.Structure):
+    _fields_ = [("ops", C.c_void_p), ("args", C.c_void_p), ("offsets", C.c_void_p),
+                ("lengths", C.c_void_p), ("n_rules", C.c_int32), ("n_nodes", C.c_int64)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(lib_path):
+        raise RuntimeError("libsigp.so not found at %s: run `python -c \"import __graft_entry__ as g; "
+                           "g.build()\"` (hipcc, gfx950) first" % lib_path)
+    L = C.CDLL(lib_path)
+    vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+    sig = {
+        "sigp_last_error": ([], C.c_char_p),
+        "sigp_device_info": ([C.c_char_p, C.c_size_t, vp, vp], C.c_int),
+        "sigp_instance_create": ([vp, vp], C.c_int),
+        "sigp_instance_destroy": ([vp], C.c_int),
+        "sigp_zstats": ([vp, vp], C.c_int),
+        "sigp_evaluate": ([vp, vp, vp, i64, i32, i32, vp, i32, i32, vp, vp], C.c_int),
+        "sigp_batch_create": ([vp, vp, i32, i32, vp, i32, i32, vp], C.c_int),
+        "sigp_batch_run": ([vp, vp, i64, vp, vp, vp], C.c_int),
+        "sigp_batch_destroy": ([vp], C.c_int),
+        "sigp_batch_shape": ([vp, vp, vp], C.c_int),
+        "sigp_device_math": ([i32, vp, vp, i64, vp], C.c_int),
+    }
+    for name, (argt, rest) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = argt
+        fn.restype = rest
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise SigpError(status, lib().sigp_last_error().decode())

@@ -1,0 +1,70 @@
+"""Named workloads: the synthetic instance, rule set and scenario set of each
+benchmark configuration, as pure functions of fixed seeds
+(instance seed 1, rules seed 0, scenarios base 1,000,000 + s).
+"""
+from __future__ import annotations
+
+import random
+from dataclasses import dataclass
+from typing import List
+
+import numpy as np
+
+from .gp import TreeConfig, ramped_half_and_half
+from .instance import Instance, synthetic_clustered, synthetic_mixed, synthetic_uniform
+from .rules import EXTRA, FUNCTIONS, NS_TERMINALS, Node, Packed, pack_trees, term_mask_from
+from .scenarios import gen_durations
+
+TEST_SEED = 1_000_000   # engine/gp_engine.cu:22
+
+__all__ = ["Workload", "plain_small", "large", "random_rules", "TEST_SEED"]
+
+
+@dataclass
+class Workload:
+    name: str
+    inst: Instance
+    trees: List[Node]
+    packed: Packed
+    mask: np.ndarray
+    sampled: np.ndarray      # S x n
+    n_scen: int
+
+    @property
+    def n_sims(self) -> int:
+        return self.packed.n_rules * self.n_scen
+
+
+def random_rules(n_rules: int, terminals, seed: int = 0, min_depth: int = 1,
+                 max_depth: int = 5, functions=FUNCTIONS) -> List[Node]:
+    """Ramped half-and-half rules (engine/gp.hpp:67-77); depths in the
+    reference convention (a leaf has depth 0), so 1..5 = 2..6 levels, <= 63 nodes."""
+    cfg = TreeConfig(list(terminals), list(functions), max_depth, min_depth)
+    return ramped_half_and_half(cfg, random.Random(seed), n_rules)
+
+
+def plain_small(n_rules: int = 1024, n_scen: int = 500, n: int = 100, m: int = 2,
+                mode: str = "gamma", shape: float = 4.0) -> Workload:
+    """1 uniform instance (n=100, m=2), EXTRA-18 rules, Gamma(4, mean 10) scenarios."""
+    inst = synthetic_uniform(n=n, m=m, seed=1)
+    trees = random_rules(n_rules, EXTRA, seed=0)
+    sampled = gen_durations(inst.durations, n_scen, TEST_SEED, mode=mode, shape=shape)
+    return Workload("plain-small n=%d m=%d %dx%d" % (n, m, n_rules, n_scen), inst, trees,
+                    pack_trees(trees), term_mask_from(EXTRA), sampled, n_scen)
+
+
+def large(n_rules: int = 4096, n_scen: int = 2000, n: int = 288, m: int = 4) -> Workload:
+    """288-customer clustered+uniform instance, m=4, rules <= 127 nodes, Gamma(2)."""
+    inst = synthetic_mixed(n=n, m=m, seed=1)
+    trees = random_rules(n_rules, EXTRA, seed=0, min_depth=1, max_depth=6)
+    sampled = gen_durations(inst.durations, n_scen, TEST_SEED, mode="gamma", shape=2.0)
+    return Workload("large n=%d m=%d %dx%d" % (n, m, n_rules, n_scen), inst, trees,
+                    pack_trees(trees), term_mask_from(EXTRA), sampled, n_scen)
+
+
+def ext29_mask() -> np.ndarray:
+    return term_mask_from(NS_TERMINALS + EXTRA)
+
+
+def clustered_gp_instance() -> Instance:
+    return synthetic_clustered(n=100, m=3, seed=1)

@@ -1,0 +1,33 @@
+"""The C ABI library loads and exports every entry point include/sigp.h
+declares (CPU only: nothing is called that needs a GPU)."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "sigp.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sigp_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_matches_binding():
+    from sigp._lib import EXPORTED
+    assert declared() == sorted(EXPORTED)
+
+
+def test_library_exports_every_symbol():
+    from sigp._lib import lib_path
+    assert os.path.exists(lib_path), "build libsigp.so first (__graft_entry__.build())"
+    L = ctypes.CDLL(lib_path)
+    for name in declared():
+        assert hasattr(L, name), name
+
+
+def test_last_error_is_empty_string_initially():
+    from sigp._lib import lib
+    assert lib().sigp_last_error() == b""
