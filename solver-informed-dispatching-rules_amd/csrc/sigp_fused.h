@@ -1,0 +1,454 @@
+// Candidate scans: the terminal families of one candidate customer c,
+// evaluated for all lanes of a wavefront (lane = scenario).
+//
+// Feasibility thresholds.  For a POI a and customer j the reference's test
+//     arr = f + d(a,j) <= close_j  and  max(arr, open_j) + dur_j + d2d_j <= tmax
+// is a monotone predicate of the departure time f (IEEE addition and max are
+// monotone), so it holds exactly for f <= tau(a,j), the largest double that
+// satisfies it (found per instance by bisection over the ordered doubles,
+// evaluating the identical expression).  Variants: tau0 (plain), tau1 (close
+// time only), tau2 (robust duration dur_j * (1 + sigma)), and tau0n / tau2n,
+// which also fold in the NS family's "skip d(a,j) < 1e-10" rule.  A terminal's
+// neighbour test then costs one compare; every value it adds or compares is
+// still the reference's own expression (engine/sim_terms.hpp).
+//
+// Each family is a short loop specialised at compile time for the members
+// the rule needs, reading eight neighbours per scalar-load chunk from rows
+// padded to a multiple of 8 (padding has tau = -inf, so it never counts).
+// Scans that start from a lane's own POI (second legs, later REACH-K
+// waypoints) gather down the columns of the transposed tables, whose rows are
+// padded the same way.
+#pragma once
+#include "sigp_terms.h"
+
+namespace sigp {
+
+// Per-rule computation flags (a second 64-bit word beside the slot bits).
+enum : uint64_t {
+    F_NS = 1ull << 0,
+    F_REGRET = 1ull << 1,
+    F_NSROB = 1ull << 2,
+    F_MAXN = 1ull << 3,
+    F_REACH2 = 1ull << 4,
+    F_NSTW = 1ull << 5,
+    F_STRUCT = 1ull << 6,
+    F_REACHK = 1ull << 7,    // greedy path REACH1/3/5 (length in bits 40-42)
+    F_ATRISK = 1ull << 8,
+    F_R2ROB = 1ull << 9,
+    F_REACHKR = 1ull << 10,  // robust path REACH3ROB/5ROB (length in bits 43-45)
+    F_FNOWROB = 1ull << 11,  // per-decision robust bits (NFEASROB)
+    F_FSCORE = 1ull << 12,   // per-decision FEASCORE sum
+    F_CNOW = 1ull << 13,     // per-decision close-only bits
+};
+
+struct FusedTabs {
+    int ldp;   // row length of [a][j] tables (>= n, multiple of 8); rows of [j][a] tables
+    int ldq;   // row length of [j][a] tables (>= n+1, multiple of 8)
+    cdp t0, t0n, t2, t2n, r, d;          // rows [a][j], a in [0, n]
+    cdp t0c, t0nc, t1c, t2c, t2nc, rc;   // columns [j][a], j in [0, ldp)
+    cdp open, close, dur, d2d, score, durrob, sdv;   // customer arrays, padded to ldp
+};
+
+// Result of the candidate scans (per lane).
+struct CandVals {
+    double ns, regret, nsrob, maxn, reach2, nstw;
+    double fcnt, fscore, dslack, pbust;
+    double rk1, rk3, rk5, rkr3, rkr5, eloss, r2rob;
+};
+
+__device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+
+struct D8 { double v[8]; };
+__device__ __forceinline__ D8 ld8(cdp row, int j0) {
+    D8 r;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r.v[k] = row[j0 + k];
+    return r;
+}
+
+// The eight bits j0..j0+7 of a per-lane bit set (j0 a multiple of 8).
+template <int NW>
+__device__ __forceinline__ uint32_t bits8(const uint32_t (&a)[NW], int j0) {
+    return (wsel<NW>(a, j0 >> 5) >> (j0 & 31)) & 0xffu;
+}
+
+// NS (engine/sim_terms.hpp:18-34), MAXN (:76-93) and the first leg of
+// REACH2 (:101-111) from candidate c at time fin, one pass.
+template <int NW, bool NS, bool MX, bool R2>
+__device__ void scan_nsfam(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW],
+                           double &ns, double &mx, double &bv, int &bj) {
+    const cdp tr = F.t0n + (size_t)(c + 1) * F.ldp, rr = F.r + (size_t)(c + 1) * F.ldp;
+    double s = 0.0, m = 0.0, b = 0.0;
+    int bi = -1;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        const uint32_t vb = bits8<NW>(vis, j0);
+        const D8 T = ld8(tr, j0), R = ld8(rr, j0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 + k;
+            const bool ok = !((vb >> k) & 1u) && j != c && fin <= T.v[k];
+            const double r = R.v[k];
+            if (NS) s = sel(ok, s + r, s);
+            if (MX) m = sel(ok && r > m, r, m);
+            if (R2) {
+                const bool take = ok && r > b;
+                b = sel(take, r, b);
+                bi = take ? j : bi;
+            }
+        }
+    }
+    ns = s; mx = m; bv = b; bj = bi;
+}
+
+// NSROB (engine/sim_terms.hpp:56-74), the first leg of REACH2ROB (:164-174)
+// and the first step of REACH-K-ROB (:455-468).
+template <int NW, bool NSR, bool R2R, bool RKR>
+__device__ void scan_robfam(const FusedTabs &F, int n, int c, double frob, double trob,
+                            const uint32_t (&vis)[NW], double &nsr, int &bjr, int &bkr) {
+    const cdp tn = F.t2n + (size_t)(c + 1) * F.ldp, tp = F.t2 + (size_t)(c + 1) * F.ldp,
+              rr = F.r + (size_t)(c + 1) * F.ldp;
+    double s = 0.0, b = 0.0, bk = -1.0;
+    int bi = -1, ki = -1;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        const uint32_t vb = bits8<NW>(vis, j0);
+        const D8 T = ld8(tn, j0), R = ld8(rr, j0);
+        D8 TP;
+        if (RKR) TP = ld8(tp, j0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 + k;
+            const bool open = !((vb >> k) & 1u) && j != c;
+            const bool ok = open && frob <= T.v[k];
+            const double r = R.v[k];
+            if (NSR) s = sel(ok, s + r, s);
+            if (R2R) {
+                const bool take = ok && r > b;
+                b = sel(take, r, b);
+                bi = take ? j : bi;
+            }
+            if (RKR) {
+                const bool take = open && trob <= TP.v[k] && r > bk;
+                bk = sel(take, r, bk);
+                ki = take ? j : ki;
+            }
+        }
+    }
+    nsr = s; bjr = bi; bkr = ki;
+}
+
+// REGRET (engine/sim_terms.hpp:36-54): neighbours feasible now (fnow) that
+// become infeasible after serving c.
+template <int NW>
+__device__ double scan_regret(const FusedTabs &F, int n, int c, double fin,
+                              const uint32_t (&vis)[NW], const uint32_t (&fnow)[NW]) {
+    const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp;
+    double s = 0.0;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        const uint32_t mb = bits8<NW>(fnow, j0) & ~bits8<NW>(vis, j0);
+        const D8 T = ld8(tr, j0), S = ld8(F.score, j0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 + k;
+            const bool add = ((mb >> k) & 1u) && j != c && !(fin <= T.v[k]);
+            s = sel(add, s + S.v[k], s);
+        }
+    }
+    return s;
+}
+
+// FRAGCNT, FRAGSCORE, DOWNSLACK and PBUST's best neighbour
+// (engine/sim_terms.hpp:544-562), after serving c at fin_c.
+template <int NW>
+__device__ void scan_struct(const FusedTabs &F, int n, int c, double fin_c, const uint32_t (&vis)[NW],
+                            const uint32_t (&fnow)[NW], double &fcnt, double &fsc, double &dsl,
+                            int &bq) {
+    const size_t row = (size_t)(c + 1) * F.ldp;
+    const cdp tr = F.t0 + row, rr = F.r + row, dd = F.d + row;
+    double cnt = 0.0, sc = 0.0, ds = 1e18, bv = -1.0;
+    int bi = -1;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        const uint32_t vb = bits8<NW>(vis, j0), fb = bits8<NW>(fnow, j0);
+        const D8 T = ld8(tr, j0), R = ld8(rr, j0), D = ld8(dd, j0), S = ld8(F.score, j0),
+                 C = ld8(F.close, j0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = j0 + k;
+            const bool open = !((vb >> k) & 1u) && q != c;
+            const bool after = fin_c <= T.v[k];
+            const bool frag = open && ((fb >> k) & 1u) && !after;
+            cnt = sel(frag, cnt + 1.0, cnt);
+            sc = sel(frag, sc + S.v[k], sc);
+            const bool aft = open && after;
+            const double slack = C.v[k] - (fin_c + D.v[k]);
+            ds = sel(aft && slack < ds, slack, ds);
+            const bool take = aft && D.v[k] > 1e-10 && R.v[k] > bv;
+            bv = sel(take, R.v[k], bv);
+            bi = take ? q : bi;
+        }
+    }
+    fcnt = cnt; fsc = sc; dsl = ds; bq = bi;
+}
+
+// First step of REACH-K (engine/sim_terms.hpp:417-431) from c at t = fin_c.
+template <int NW>
+__device__ int scan_reach1(const FusedTabs &F, int n, int c, double t, const uint32_t (&vis)[NW]) {
+    const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp, rr = F.r + (size_t)(c + 1) * F.ldp;
+    double bv = -1.0;
+    int bi = -1;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        const uint32_t vb = bits8<NW>(vis, j0);
+        const D8 T = ld8(tr, j0), R = ld8(rr, j0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int q = j0 + k;
+            const bool take = !((vb >> k) & 1u) && q != c && t <= T.v[k] && R.v[k] > bv;
+            bv = sel(take, R.v[k], bv);
+            bi = take ? q : bi;
+        }
+    }
+    return bi;
+}
+
+// NSTW (engine/sim_terms.hpp:192-213).  The weight is exactly 1 unless the
+// lane arrives before the window opens, so the common case is a plain
+// add of r (r * 1.0 == r); the division path runs one neighbour at a time.
+template <int NW>
+__device__ double scan_nstw(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW]) {
+    const size_t row = (size_t)(c + 1) * F.ldp;
+    const cdp tr = F.t0n + row, rr = F.r + row, dd = F.d + row;
+    double s = 0.0;
+    for (int j0 = 0; j0 < n; j0 += 8) {
+        const uint32_t vb = bits8<NW>(vis, j0);
+        const D8 T = ld8(tr, j0);
+        uint32_t okm = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const bool ok = !((vb >> k) & 1u) && j0 + k != c && fin <= T.v[k];
+            okm |= (ok ? 1u : 0u) << k;
+        }
+        if (__builtin_amdgcn_ballot_w64(okm != 0) == 0) continue;
+#pragma unroll 1
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 + k;
+            if (!((okm >> k) & 1u)) continue;
+            const double r = rr[j], o = F.open[j], cl = F.close[j];
+            double wait = o - (fin + dd[j]);
+            if (wait < 0.0) wait = 0.0;
+            const double win = cl - o;
+            double w = win > 1e-10 ? 1.0 - wait / win : 1.0;
+            if (w < 0.0) w = 0.0;
+            s += r * w;
+        }
+    }
+    return s;
+}
+
+// ATRISK (engine/sim_terms.hpp:225-257) for candidate c.  pow(var, 0.5)
+// comes from the per-instance duration-class table.  One neighbour at a
+// time: the body holds two erf evaluations.
+template <int NW>
+__device__ double scan_eloss(const DevInst &I, const FusedTabs &F, int c, double fin_c,
+                             const uint32_t (&vis)[NW], const uint32_t (&cnow)[NW]) {
+    const double sdc = F.sdv[c];
+    const double sdc2 = sdc * sdc;
+    const cdp pwrow = I.pw2 + (size_t)I.dcls[c] * I.K;
+    const cdp dd = F.d + (size_t)(c + 1) * F.ldp;
+    double el = 0.0;
+    uint32_t mb = 0;
+#pragma unroll 1
+    for (int j = 0; j < I.n; ++j) {
+        if ((j & 31) == 0) mb = wsel<NW>(cnow, j >> 5) & ~wsel<NW>(vis, j >> 5);
+        const bool act = j != c && ((mb >> (j & 31)) & 1u);
+        if (__builtin_amdgcn_ballot_w64(act) == 0) continue;
+        if (!act) continue;
+        const double ma = fin_c + dd[j];
+        const double cl = F.close[j];
+        const double pw = sdc > 1e-9 ? norm_cdf((cl - ma) / sdc) : (ma <= cl ? 1.0 : 0.0);
+        double pb = 1.0;
+        if (pw != 0.0) {   // 1 - 0 * pb == 1 for every pb in [0, 1]
+            const double op = F.open[j];
+            const bool late = ma > op;
+            const double ms = late ? ma : op;
+            const double vs = late ? sdc2 : 0.0;
+            const double sdj = F.sdv[j];
+            const double vb = vs + sdj * sdj;
+            const double mbd = ms + F.dur[j] + F.d2d[j];
+            if (vb > 1e-18) {
+                const int kj = I.dcls[j];
+                pb = norm_cdf((I.tmax - mbd) / (late ? pwrow[kj] : I.pw1[kj]));
+            } else {
+                pb = mbd <= I.tmax ? 1.0 : 0.0;
+            }
+        }
+        el += F.score[j] * (1.0 - pw * pb);
+    }
+    return el;
+}
+
+// Second leg of REACH2 / REACH2ROB (engine/sim_terms.hpp:113-126 and
+// :176-189) from each lane's own first neighbour bj at time fj.
+template <int NW>
+__device__ double reach2_leg2(const DevInst &I, const FusedTabs &F, int i, int bj, double fj,
+                              const uint32_t (&vis)[NW], bool rob) {
+    const int pj = bj + 1;
+    const cdp tcol = rob ? F.t2nc : F.t0nc;
+    double bv2 = 0.0, s2 = 0.0;
+    for (int k0 = 0; k0 < I.n; k0 += 4) {
+        const uint32_t vb = (wsel<NW>(vis, k0 >> 5) >> (k0 & 31)) & 0xfu;
+        double T[4], R[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t at = (size_t)(k0 + u) * F.ldq + pj;
+            T[u] = tcol[at];
+            R[u] = F.rc[at];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u;
+            const bool take = k != i && k != bj && !((vb >> u) & 1u) && fj <= T[u] && R[u] > bv2;
+            bv2 = sel(take, R[u], bv2);
+            s2 = sel(take, F.score[k], s2);
+        }
+    }
+    return I.score[bj] + s2;
+}
+
+// The REACH-K greedy path (engine/sim_terms.hpp:405-441; robust :443-479)
+// given its first waypoint; later waypoints scan the lane's own row.
+// REACH1/3/5 are prefixes of one path: returns the totals after
+// min(K, picks) for K = 1, 3, 5.
+template <int NW>
+__device__ void reach_path(const DevInst &I, const FusedTabs &F, int c, int best1, double t0,
+                           const uint32_t (&vis)[NW], int K, bool rob, double *tot1,
+                           double *tot3, double *tot5) {
+    double total = 0.0;
+    *tot1 = *tot3 = *tot5 = 0.0;
+    if (best1 < 0) return;
+    const cdp tcol = rob ? F.t2c : F.t0c;
+    uint32_t scr[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) scr[w] = vis[w];
+    wset<NW>(scr, c);
+    int cur = c + 1;
+    double t = t0;
+    int best = best1;
+    for (int step = 0; step < K; ++step) {
+        if (step > 0) {
+            best = -1;
+            double bv = -1.0;
+            for (int q0 = 0; q0 < I.n; q0 += 4) {
+                const uint32_t sb = (wsel<NW>(scr, q0 >> 5) >> (q0 & 31)) & 0xfu;
+                double T[4], R[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const size_t at = (size_t)(q0 + u) * F.ldq + cur;
+                    T[u] = tcol[at];
+                    R[u] = F.rc[at];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const bool take = !((sb >> u) & 1u) && t <= T[u] && R[u] > bv;
+                    bv = sel(take, R[u], bv);
+                    best = take ? q0 + u : best;
+                }
+            }
+            if (best < 0) break;
+        }
+        const double ta = F.d[(size_t)cur * F.ldp + best];
+        const double arr = t + ta;
+        const double st = arr > F.open[best] ? arr : F.open[best];
+        t = rob ? st + F.durrob[best] : st + F.dur[best];
+        cur = best + 1;
+        total += F.score[best];
+        wset<NW>(scr, best);
+        // REACH-K stops after K picks or when no waypoint is left, so its
+        // value is the running total after min(K, picks) steps
+        if (step == 0) *tot1 = total;
+        if (step <= 2) *tot3 = total;
+        *tot5 = total;
+    }
+}
+
+// Every scan the rule needs for candidate c.  fin = ct + tfv (NS family),
+// st_c = max(ct + ta, open_c), fin_c = st_c + dur_c (structural, REACH-K,
+// ATRISK).
+template <int NW>
+__device__ void cand_scans(const DevInst &I, const FusedTabs &F, uint64_t fl, int kpath, int kpathr,
+                           int c, double fin, double st_c, double fin_c, const uint32_t (&vis)[NW],
+                           const uint32_t (&fnow)[NW], const uint32_t (&cnow)[NW], CandVals &o) {
+    const int n = I.n;
+    const double dur_c = F.dur[c];
+    o.ns = o.maxn = o.reach2 = 0.0;
+    double bv2 = 0.0;
+    int bj2 = -1;
+    switch ((int)(fl & (F_NS | F_MAXN | F_REACH2))) {
+    case (int)F_NS: scan_nsfam<NW, true, false, false>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    case (int)F_MAXN: scan_nsfam<NW, false, true, false>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    case (int)F_REACH2: scan_nsfam<NW, false, false, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    case (int)(F_NS | F_MAXN): scan_nsfam<NW, true, true, false>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    case (int)(F_NS | F_REACH2): scan_nsfam<NW, true, false, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    case (int)(F_MAXN | F_REACH2): scan_nsfam<NW, false, true, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    case (int)(F_NS | F_MAXN | F_REACH2): scan_nsfam<NW, true, true, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
+    default: break;
+    }
+    if ((fl & F_REACH2) && bj2 >= 0) {   // engine/sim_terms.hpp:110,112-126
+        const double arr = fin + F.d[(size_t)(c + 1) * F.ldp + bj2];
+        const double st = arr > F.open[bj2] ? arr : F.open[bj2];
+        o.reach2 = reach2_leg2<NW>(I, F, c, bj2, st + F.dur[bj2], vis, false);
+    }
+
+    const double frob = fin + dur_c * kSigma;   // engine/sim_terms.hpp:61,161
+    const double trob = st_c + F.durrob[c];     // engine/sim_terms.hpp:453
+    o.nsrob = o.r2rob = 0.0;
+    int bjr = -1, bkr = -1;
+    switch ((int)(fl & (F_NSROB | F_R2ROB | F_REACHKR))) {
+    case (int)F_NSROB: scan_robfam<NW, true, false, false>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    case (int)F_R2ROB: scan_robfam<NW, false, true, false>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    case (int)F_REACHKR: scan_robfam<NW, false, false, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    case (int)(F_NSROB | F_R2ROB): scan_robfam<NW, true, true, false>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    case (int)(F_NSROB | F_REACHKR): scan_robfam<NW, true, false, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    case (int)(F_R2ROB | F_REACHKR): scan_robfam<NW, false, true, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    case (int)(F_NSROB | F_R2ROB | F_REACHKR): scan_robfam<NW, true, true, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
+    default: break;
+    }
+    if ((fl & F_R2ROB) && bjr >= 0) {   // engine/sim_terms.hpp:173,175-189
+        const double arr = frob + F.d[(size_t)(c + 1) * F.ldp + bjr];
+        const double st = arr > F.open[bjr] ? arr : F.open[bjr];
+        o.r2rob = reach2_leg2<NW>(I, F, c, bjr, st + F.durrob[bjr], vis, true);
+    }
+    o.rkr3 = o.rkr5 = 0.0;
+    if (fl & F_REACHKR) {
+        double t1;
+        reach_path<NW>(I, F, c, bkr, trob, vis, kpathr, true, &t1, &o.rkr3, &o.rkr5);
+    }
+
+    o.regret = (fl & F_REGRET) ? scan_regret<NW>(F, n, c, fin, vis, fnow) : 0.0;
+
+    o.fcnt = o.fscore = o.dslack = o.pbust = 0.0;
+    if (fl & F_STRUCT) {
+        int bq = -1;
+        double dsl;
+        scan_struct<NW>(F, n, c, fin_c, vis, fnow, o.fcnt, o.fscore, dsl, bq);
+        o.dslack = dsl > 1e17 ? 0.0 : dsl;
+        if (bq >= 0) {   // engine/sim_terms.hpp:565-575
+            const double ab = fin_c + F.d[(size_t)(c + 1) * F.ldp + bq];
+            const double sd = kSigma * dur_c;
+            if (sd > 1e-12) {
+                const double z = (F.close[bq] - ab) / (sd * kSqrt2);
+                o.pbust = 0.5 * sigp_erfc(z);
+            } else {
+                o.pbust = ab > F.close[bq] ? 1.0 : 0.0;
+            }
+        }
+    }
+
+    o.rk1 = o.rk3 = o.rk5 = 0.0;
+    if (fl & F_REACHK) {
+        const int b1 = scan_reach1<NW>(F, n, c, fin_c, vis);
+        reach_path<NW>(I, F, c, b1, fin_c, vis, kpath, false, &o.rk1, &o.rk3, &o.rk5);
+    }
+    o.nstw = (fl & F_NSTW) ? scan_nstw<NW>(F, n, c, fin, vis) : 0.0;
+    o.eloss = (fl & F_ATRISK) ? scan_eloss<NW>(I, F, c, fin_c, vis, cnow) : 0.0;
+}
+
+}  // namespace sigp

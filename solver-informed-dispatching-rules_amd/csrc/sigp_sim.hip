@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,7 @@
 #include "../../include/sigp.h"
 #include "sigp_libm.h"
 #include "sigp_terms.h"
+#include "sigp_fused.h"
 
 using namespace sigp;
 
@@ -53,10 +55,6 @@ static sigp_status fail(sigp_status st, const std::string &msg) {
 
 extern "C" const char *sigp_last_error(void) { return g_err.c_str(); }
 
-// Internal bits of a rule's compute set (slots use bits 0..60).
-constexpr uint64_t kNeedNs = 1ull << 61;       // ns_val (engine/sim_core.hpp:100-102)
-constexpr uint64_t kNeedStruct = 1ull << 62;   // struct_feats_one (:116-121)
-constexpr uint64_t kNeedPre = 1ull << 63;      // VFRAC/NFEAS pre-pass (:57-71)
 constexpr uint64_t B(int s) { return 1ull << s; }
 
 // ---------------------------------------------------------------------------
@@ -80,6 +78,79 @@ __global__ void prep_tables_kernel(int n, const double *__restrict__ dist,
             rk[(size_t)a * n + q] = v;
             rkT[(size_t)q * ld + a] = v;
         }
+    }
+}
+
+// Order-preserving map from doubles to unsigned keys and back.
+__device__ __forceinline__ uint64_t okey(double x) {
+    const uint64_t u = sigp_bits(x);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double unkey(uint64_t k) {
+    return sigp_dbl((k >> 63) ? (k & 0x7fffffffffffffffull) : ~k);
+}
+
+// Largest finite double f with pred(f) true for a predicate that is true
+// below some point and false above it; -inf when it is never true.
+template <class Pred>
+__device__ double last_true(Pred pred) {
+    const double lo_v = -1.7976931348623157e308, hi_v = 1.7976931348623157e308;
+    if (!pred(lo_v)) return -__builtin_inf();
+    if (pred(hi_v)) return hi_v;
+    uint64_t lo = okey(lo_v), hi = okey(hi_v);
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (pred(unkey(mid))) lo = mid;
+        else hi = mid;
+    }
+    return unkey(lo);
+}
+
+// Feasibility thresholds and ratios per (POI a, customer j): rows [a][j]
+// (row length ldp) and columns [j][a] (ldp rows of length ldq; rows j >= n
+// are padding).  The predicates are the reference's neighbour tests verbatim
+// (engine/sim_terms.hpp:27-30, 67-70, 522-531): f + d > close fails;
+// max(f + d, open) + dur + d2d > tmax fails (dur * (1 + sigma) for the
+// robust variant); the NS-family variants also fail when d < 1e-10.
+__global__ void prep_thresholds_kernel(int n, int ldp, int ldq, const double *__restrict__ dist,
+                                       const double *__restrict__ open, const double *__restrict__ close,
+                                       const double *__restrict__ dur, const double *__restrict__ durrob,
+                                       const double *__restrict__ d2d, const double *__restrict__ score,
+                                       double tmax, double *t0, double *t0n, double *t2, double *t2n,
+                                       double *r, double *d, double *t0c, double *t0nc, double *t1c,
+                                       double *t2c, double *t2nc, double *rc) {
+    const int ld = n + 1;
+    const long long total = (long long)ld * ldp;
+    const double ninf = -__builtin_inf();
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)gridDim.x * blockDim.x) {
+        const int a = (int)(idx / ldp), j = (int)(idx % ldp);
+        double v0 = ninf, v1 = ninf, v2 = ninf, vr = 0.0, vd = __builtin_inf();
+        bool near = true;
+        if (j < n) {
+            const double dd = dist[(size_t)a * ld + j + 1];
+            const double cl = close[j], op = open[j], du = dur[j], dr = durrob[j], dd2 = d2d[j];
+            v1 = last_true([&](double f) { return !(f + dd > cl); });
+            v0 = last_true([&](double f) {
+                const double arr = f + dd;
+                if (arr > cl) return false;
+                const double st = arr > op ? arr : op;
+                return !(st + du + dd2 > tmax);
+            });
+            v2 = last_true([&](double f) {
+                const double arr = f + dd;
+                if (arr > cl) return false;
+                const double st = arr > op ? arr : op;
+                return !(st + dr + dd2 > tmax);
+            });
+            vr = score[j] / (dd > 1e-10 ? dd : 1e-10);
+            vd = dd;
+            near = dd < 1e-10;
+        }
+        const double v0n = near ? ninf : v0, v2n = near ? ninf : v2;
+        t0[idx] = v0; t0n[idx] = v0n; t2[idx] = v2; t2n[idx] = v2n; r[idx] = vr; d[idx] = vd;
+        const size_t c = (size_t)j * ldq + a;
+        t0c[c] = v0; t0nc[c] = v0n; t1c[c] = v1; t2c[c] = v2; t2nc[c] = v2n; rc[c] = vr;
     }
 }
 
@@ -150,8 +221,9 @@ __global__ void zreduce_kernel(int n, const double *__restrict__ feat, const int
 
 struct RuleSet {
     cip ops, args, off, len;
-    cup comp;    // per rule: slots to compute + internal bits 61..63
-    cup mbits;   // per rule: its terminal-mask row as bits
+    cup comp;    // per rule: slots whose value the rule reads
+    cup flags;   // per rule: fused computations (F_*) and REACH path lengths
+    cip order;   // launch order of (rule, scenario block) items, or null
     int P;
 };
 
@@ -198,53 +270,64 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
 }
 
 // One decision of pick_best (engine/sim_core.hpp:43-204) for every running lane.
+// Candidates i0, i0 + istep, ... of one decision; returns this subset's
+// first maximum (index, priority) -- the reference's strict ">" scan in index
+// order -- so that subsets evaluated by cooperating wavefronts merge exactly.
 template <int NW, int STK>
-__device__ int decide(const DevInst &I, const cip ops, const cip args, int L, uint64_t comp,
-                      uint64_t mk, int cur, double ct, int rem, const uint32_t (&vis)[NW]) {
-    uint32_t fnow[NW], cnow[NW];
+__device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
+                      uint64_t comp, uint64_t fl, int cur, double ct, int rem,
+                      const uint32_t (&vis)[NW], int i0, int istep, double &best_pri_out) {
+    // Per-decision feasibility from the lane's current POI: fnow (the
+    // candidate test, :78-82), cnow (close time only, ATRISK :239-240) and
+    // the robust bits of NFEASROB (:68); thresholds read down column `cur`.
+    uint32_t fnow[NW], cnow[NW], frob[NW];
 #pragma unroll
-    for (int w = 0; w < NW; ++w) { fnow[w] = 0; cnow[w] = 0; }
-    double nvis = 0.0, nfeas = 0.0, fsc = 0.0, nfr = 0.0;
-    const bool pre = (comp & kNeedPre) != 0;
+    for (int w = 0; w < NW; ++w) { fnow[w] = 0; cnow[w] = 0; frob[w] = 0; }
+    const bool want_c = fl & F_CNOW, want_r = fl & F_FNOWROB;
     {
-        uint32_t vw = 0, fw = 0, cw = 0;
+        uint32_t fw = 0, cw = 0, rw = 0;
         for (int j = 0; j < I.n; ++j) {
             const int bit = j & 31;
-            if (bit == 0) { vw = wsel<NW>(vis, j >> 5); fw = 0; cw = 0; }
-            const bool visited = (vw >> bit) & 1u;
-            const double ta = I.distT[(size_t)(j + 1) * I.ld + cur];
-            const double a = ct + ta;
-            if (!(a > I.close[j])) {
-                cw |= 1u << bit;
-                const double st = a > I.open[j] ? a : I.open[j];
-                if (!(st + I.dur[j] + I.d2d[j] > I.tmax)) {
-                    fw |= 1u << bit;
-                    if (pre && !visited) {
-                        nfeas += 1.0;
-                        fsc += I.score[j];
-                        if (st + I.durrob[j] + I.d2d[j] <= I.tmax) nfr += 1.0;
-                    }
-                }
-            }
-            if (pre && visited) nvis += 1.0;
+            const size_t at = (size_t)j * F.ldq + cur;
+            fw |= (ct <= F.t0c[at] ? 1u : 0u) << bit;
+            if (want_c) cw |= (ct <= F.t1c[at] ? 1u : 0u) << bit;
+            if (want_r) rw |= (ct <= F.t2c[at] ? 1u : 0u) << bit;
             if (bit == 31 || j == I.n - 1) {
                 const int w = j >> 5;
 #pragma unroll
                 for (int k = 0; k < NW; ++k) {
                     fnow[k] = (w == k) ? fw : fnow[k];
                     cnow[k] = (w == k) ? cw : cnow[k];
+                    frob[k] = (w == k) ? rw : frob[k];
                 }
+                fw = cw = rw = 0;
             }
         }
     }
-    const double vfrac = pre ? nvis / I.n : 0.0;
+    // VFRAC / NFEAS / FEASCORE / NFEASROB pre-pass (:56-71): counts are exact
+    // integers; FEASCORE sums scores in customer order.
+    double nvis = 0.0, nfeas = 0.0, fsc = 0.0, nfr = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        nvis += (double)__builtin_popcount(vis[w]);
+        nfeas += (double)__builtin_popcount(fnow[w] & ~vis[w]);
+        nfr += (double)__builtin_popcount(fnow[w] & frob[w] & ~vis[w]);
+    }
+    if (fl & F_FSCORE) {
+        uint32_t m = 0;
+        for (int j = 0; j < I.n; ++j) {
+            if ((j & 31) == 0) m = wsel<NW>(fnow, j >> 5) & ~wsel<NW>(vis, j >> 5);
+            if ((m >> (j & 31)) & 1u) fsc += I.score[j];
+        }
+    }
+    const double vfrac = nvis / I.n;
     const double sat = I.tmax > 0.0 ? ct / I.tmax : 0.0;
+    const int kpath = (int)((fl >> 40) & 7), kpathr = (int)((fl >> 43) & 7);
 
     int best = -1;
     double best_pri = -1e300;
     double T[SIGP_N_TERMS];
-    double sbuf[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = 0; i < I.n; ++i) {
+    for (int i = i0; i < I.n; i += istep) {
         const uint32_t b = 1u << (i & 31);
         const int w = i >> 5;
         const bool feas = !(wsel<NW>(vis, w) & b) && (wsel<NW>(fnow, w) & b);
@@ -271,30 +354,29 @@ __device__ int decide(const DevInst &I, const cip ops, const cip args, int L, ui
         const double st_c = arr > oi ? arr : oi;
         const double fin_c = st_c + di;
 
-        double nsv = 0.0;
-        if (comp & kNeedNs) nsv = t_ns<NW>(I, i, fin, vis);
-        if (comp & B(10)) T[10] = nsv;
-        if (comp & B(11)) T[11] = t_regret<NW>(I, i, fin, vis, fnow);
-        if (comp & B(12)) T[12] = t_ns_rob_m<NW>(I, i, fin, di, vis, 1.0, false);
-        if (comp & B(13)) T[13] = t_maxn<NW>(I, i, fin, vis);
-        if (comp & B(14)) T[14] = t_reach2x<NW>(I, i, fin, 0.0, vis, false);
-        if (comp & B(15)) T[15] = t_nstw<NW>(I, i, fin, vis);
+        CandVals v{};
+        if (fl & 0x7FFull) cand_scans<NW>(I, F, fl, kpath, kpathr, i, fin, st_c, fin_c, vis, fnow, cnow, v);
+        if (comp & B(10)) T[10] = v.ns;
+        if (comp & B(11)) T[11] = v.regret;
+        if (comp & B(12)) T[12] = v.nsrob;
+        if (comp & B(13)) T[13] = v.maxn;
+        if (comp & B(14)) T[14] = v.reach2;
+        if (comp & B(15)) T[15] = v.nstw;
         if (comp & B(16)) T[16] = t_rand(i);
-        if (comp & kNeedStruct) struct_feats<NW>(I, i, fin_c, vis, fnow, sbuf);
-        if (comp & B(17)) T[17] = sbuf[0];
-        if (comp & B(18)) T[18] = sbuf[1];
-        if (comp & B(19)) T[19] = sbuf[2];
-        if (comp & B(20)) T[20] = sbuf[3];
-        if (comp & B(21)) T[21] = t_reach_k<NW>(I, i, st_c, vis, 1, false);
-        if (comp & B(22)) T[22] = t_reach_k<NW>(I, i, st_c, vis, 3, false);
-        if (comp & B(23)) T[23] = t_reach_k<NW>(I, i, st_c, vis, 5, false);
+        if (comp & B(17)) T[17] = v.fcnt;
+        if (comp & B(18)) T[18] = v.fscore;
+        if (comp & B(19)) T[19] = v.dslack;
+        if (comp & B(20)) T[20] = v.pbust;
+        if (comp & B(21)) T[21] = v.rk1;
+        if (comp & B(22)) T[22] = v.rk3;
+        if (comp & B(23)) T[23] = v.rk5;
         if (comp & B(24)) T[24] = sat;
         if (comp & B(25)) T[25] = vfrac;
         if (comp & B(26)) T[26] = nfeas;
         if (comp & B(27)) T[27] = (double)rem;
-        if (comp & B(28)) T[28] = sbuf[1] / (nsv + 1.0);
-        if (comp & B(29)) T[29] = sbuf[1] - nsv;
-        if (comp & B(30)) T[30] = t_eloss<NW>(I, i, fin_c, vis, cnow);
+        if (comp & B(28)) T[28] = v.fscore / (v.ns + 1.0);
+        if (comp & B(29)) T[29] = v.fscore - v.ns;
+        if (comp & B(30)) T[30] = v.eloss;
         if (comp & B(31)) T[31] = (T[3] - I.zp[0]) / I.zp[5];
         if (comp & B(32)) T[32] = (T[14] - I.zp[1]) / I.zp[6];
         if (comp & B(33)) T[33] = (T[11] - I.zp[2]) / I.zp[7];
@@ -305,24 +387,22 @@ __device__ int decide(const DevInst &I, const cip ops, const cip args, int L, ui
         if (comp & B(38)) T[38] = 0.3;
         if (comp & B(39)) T[39] = 0.4;
         if (comp & B(40)) T[40] = 0.5;
-        if (comp & B(41)) T[41] = t_reach2x<NW>(I, i, fin, di, vis, true);
+        if (comp & B(41)) T[41] = v.r2rob;
         if (comp & B(42)) {
             double den = I.tmax - ct;
             if (den < 1.0) den = 1.0;
             T[42] = (ta + di) / den;
         }
-        if (comp & B(43))
-            T[43] = t_reach2x<NW>(I, i, fin, di, vis, true) - t_regret<NW>(I, i, fin, vis, fnow);
+        if (comp & B(43)) T[43] = v.r2rob - v.regret;
         if (comp & B(44)) T[44] = t_skiploss<NW>(I, i, cur, ct, vis);
-        if (comp & B(45))
-            T[45] = t_ns<NW>(I, i, fin, vis) - t_ns_rob_m<NW>(I, i, fin, di, vis, 1.0, false);
+        if (comp & B(45)) T[45] = v.ns - v.nsrob;
         if (comp & B(46)) {
             double den = ta + I.d2d[i];
             if (den < 1.0) den = 1.0;
-            T[46] = (I.score[i] + t_ns_rob_m<NW>(I, i, fin, di, vis, 1.0, false)) / den;
+            T[46] = (I.score[i] + v.nsrob) / den;
         }
-        if (comp & B(47)) T[47] = t_reach_k<NW>(I, i, st_c, vis, 3, true);
-        if (comp & B(48)) T[48] = t_reach_k<NW>(I, i, st_c, vis, 5, true);
+        if (comp & B(47)) T[47] = v.rkr3;
+        if (comp & B(48)) T[48] = v.rkr5;
         if (comp & B(49)) T[49] = t_ns_alpha<NW>(I, i, fin, vis, false);
         if (comp & B(50)) T[50] = t_ns_alpha<NW>(I, i, fin, vis, true);
         if (comp & B(51)) T[51] = fsc;
@@ -335,31 +415,39 @@ __device__ int decide(const DevInst &I, const cip ops, const cip args, int L, ui
         if (comp & B(58)) T[58] = I.distT[cur] + I.d2d[i] - ta;   // dist[cur][0]
         if (comp & B(59)) T[59] = t_detour<NW>(I, i, cur, fin, ta, vis);
         if (comp & B(60)) T[60] = t_scorerank<NW>(I, i, fin, vis);
-        (void)mk;
 
         double pri = eval_rule<STK>(ops, args, L, T);
         if (pri != pri) pri = -1e300;
         if (pri > best_pri) { best_pri = pri; best = i; }
     }
+    best_pri_out = best_pri;
     return best;
 }
 
-// One wave = one rule x 64 scenarios (engine/sim_core.hpp:207-238 per lane).
+// One workgroup = one rule x a block of 64 scenarios (engine/sim_core.hpp:
+// 207-238 per lane), simulated by W = blockDim.x / 64 cooperating
+// wavefronts: wavefront w evaluates candidates w, w + W, ... of every
+// decision, and the W partial first-maxima are merged through LDS (largest
+// priority, then lowest index), which is exactly the reference's argmax.  All
+// W wavefronts hold identical route state, so they leave the loop together.
 template <int NW, int STK>
-__global__ __launch_bounds__(256) void sim_kernel(DevInst I, RuleSet R, ScenSpec S,
+__global__ __launch_bounds__(256) void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
                                                   double *__restrict__ out) {
-    const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    __shared__ double lpri[4][64];
+    __shared__ int lidx[4][64];
+    const int item = R.order ? R.order[blockIdx.x] : (int)blockIdx.x;
+    const int W = (int)(blockDim.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const int p = wave / S.nblk;
-    if (p >= R.P) return;
-    const int blk = wave - p * S.nblk;
+    const int p = item / S.nblk;
+    const int blk = item - p * S.nblk;
     const int s = blk * 64 + lane;
     const bool active = s < S.T;
     const int off = R.off[p], L = R.len[p];
     const cip ops = R.ops + off;
     const cip args = R.args + 2 * (size_t)off;
     const uint64_t comp = R.comp[p];
-    const uint64_t mk = R.mbits[p];
+    const uint64_t fl = R.flags[p];
     const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
     const double *__restrict__ samp = S.sampled + row * (size_t)I.n;
 
@@ -370,9 +458,27 @@ __global__ __launch_bounds__(256) void sim_kernel(DevInst I, RuleSet R, ScenSpec
     double ct = 0.0, total = 0.0;
     bool running = active && I.m > 0;
     while (__builtin_amdgcn_ballot_w64(running) != 0) {
+        int best = -1;
+        double bp = -1e300;
         if (running) {
             const int rem = I.m - veh - 1;
-            const int best = decide<NW, STK>(I, ops, args, L, comp, mk, cur, ct, rem, vis);
+            best = decide<NW, STK>(I, F, ops, args, L, comp, fl, cur, ct, rem, vis, wv, W, bp);
+        }
+        if (W > 1) {
+            lpri[wv][lane] = bp;
+            lidx[wv][lane] = best;
+            __syncthreads();
+            double m = -1e300;
+            int bi = -1;
+            for (int w = 0; w < W; ++w) {
+                const int ix = lidx[w][lane];
+                const double pr = lpri[w][lane];
+                if (ix >= 0 && (pr > m || (pr == m && ix < bi))) { m = pr; bi = ix; }
+            }
+            best = bi;
+            __syncthreads();
+        }
+        if (running) {
             if (best < 0) {
                 ++veh;
                 cur = 0;
@@ -390,7 +496,7 @@ __global__ __launch_bounds__(256) void sim_kernel(DevInst I, RuleSet R, ScenSpec
             }
         }
     }
-    if (active) out[(size_t)p * S.T + s] = total;
+    if (active && wv == 0) out[(size_t)p * S.T + s] = total;
 }
 
 // Per-rule mean in scenario order, then / T (engine/gp_engine.cu:76-81).
@@ -427,7 +533,9 @@ struct sigp_instance {
     int n = 0, m = 0, K = 0;
     double tmax = 0.0;
     void *dbuf = nullptr;           // one device allocation holding every table
+    void *tbuf = nullptr;           // feasibility-threshold tables
     DevInst dev{};
+    FusedTabs tabs{};
     std::vector<double> zp;
     double *scen_buf = nullptr;     // cached staging for sigp_evaluate
     size_t scen_cap = 0;
@@ -440,6 +548,7 @@ struct sigp_instance {
 struct sigp_batch {
     sigp_instance *inst = nullptr;
     int P = 0, T = 0, group_size = 1, stk = 16;
+    int waves_per_item = 4;         // cooperating wavefronts per (rule, scenario block)
     int64_t rows_needed = 0;
     void *dbuf = nullptr;
     RuleSet rs{};
@@ -541,17 +650,19 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     }
 
     const size_t nn = (size_t)ld * ld;
+    const size_t np = (size_t)std::max(8, (n + 7) / 8 * 8);   // customer arrays padded for 4-wide reads
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes); return o; };
     const size_t o_dist = take(nn * 8), o_distT = take(nn * 8), o_rk = take((size_t)ld * n * 8 + 8),
-                 o_rkT = take((size_t)n * ld * 8 + 8), o_open = take(n * 8 + 8),
-                 o_close = take(n * 8 + 8), o_dur = take(n * 8 + 8), o_d2d = take(n * 8 + 8),
-                 o_score = take(n * 8 + 8), o_durrob = take(n * 8 + 8), o_sdv = take(n * 8 + 8),
+                 o_rkT = take((size_t)n * ld * 8 + 8), o_open = take(np * 8),
+                 o_close = take(np * 8), o_dur = take(np * 8), o_d2d = take(np * 8),
+                 o_score = take(np * 8), o_durrob = take(np * 8), o_sdv = take(np * 8),
                  o_cls = take(n * 4 + 8), o_pw2 = take((size_t)K * K * 8), o_pw1 = take(K * 8),
                  o_zp = take(10 * 8), o_feat = take((size_t)n * 5 * 8 + 8), o_ok = take(n * 4 + 8);
     char *base = nullptr;
     HIP_TRY(hipMalloc((void **)&base, off));
     I->dbuf = base;
+    HIP_TRY(hipMemset(base, 0, off));
     auto up = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
         return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
     };
@@ -599,6 +710,35 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
                                             (double *)(base + o_rk), (double *)(base + o_rkT));
         HIP_TRY(hipGetLastError());
     }
+    {
+        FusedTabs &F = I->tabs;
+        F.ldp = (int)np;
+        F.ldq = (n + 1 + 7) / 8 * 8;
+        const size_t rows = (size_t)ld * F.ldp, cols = (size_t)F.ldp * F.ldq;
+        char *tb = nullptr;
+        HIP_TRY(hipMalloc((void **)&tb, 6 * (rows + cols) * 8));
+        I->tbuf = tb;
+        double *p = (double *)tb;
+        double *t[12];
+        for (int k = 0; k < 6; ++k) { t[k] = p; p += rows; }
+        for (int k = 6; k < 12; ++k) { t[k] = p; p += cols; }
+        const long long total = (long long)rows;
+        int blocks = (int)std::min<long long>((total + 255) / 256, 8192);
+        if (blocks < 1) blocks = 1;
+        prep_thresholds_kernel<<<blocks, 256>>>(
+            n, F.ldp, F.ldq, (const double *)(base + o_dist), (const double *)(base + o_open),
+            (const double *)(base + o_close), (const double *)(base + o_dur),
+            (const double *)(base + o_durrob), (const double *)(base + o_d2d),
+            (const double *)(base + o_score), d->tmax, t[0], t[1], t[2], t[3], t[4], t[5], t[6],
+            t[7], t[8], t[9], t[10], t[11]);
+        HIP_TRY(hipGetLastError());
+        F.t0 = (cdp)t[0]; F.t0n = (cdp)t[1]; F.t2 = (cdp)t[2]; F.t2n = (cdp)t[3];
+        F.r = (cdp)t[4]; F.d = (cdp)t[5];
+        F.t0c = (cdp)t[6]; F.t0nc = (cdp)t[7]; F.t1c = (cdp)t[8]; F.t2c = (cdp)t[9];
+        F.t2nc = (cdp)t[10]; F.rc = (cdp)t[11];
+        F.open = D.open; F.close = D.close; F.dur = D.dur; F.d2d = D.d2d; F.score = D.score;
+        F.durrob = D.durrob; F.sdv = D.sdv;
+    }
     I->zp.assign(10, 0.0);
     if (d->zp) {
         std::memcpy(I->zp.data(), d->zp, 80);
@@ -630,6 +770,7 @@ extern "C" sigp_status sigp_instance_create(const sigp_instance_desc *desc, sigp
     st = build_instance(desc, I);
     if (st != SIGP_OK) {
         if (I->dbuf) (void)hipFree(I->dbuf);
+        if (I->tbuf) (void)hipFree(I->tbuf);
         delete I;
         return st;
     }
@@ -640,6 +781,7 @@ extern "C" sigp_status sigp_instance_create(const sigp_instance_desc *desc, sigp
 extern "C" sigp_status sigp_instance_destroy(sigp_instance *inst) {
     if (!inst) return SIGP_OK;
     if (inst->dbuf) (void)hipFree(inst->dbuf);
+    if (inst->tbuf) (void)hipFree(inst->tbuf);
     if (inst->scen_buf) (void)hipFree(inst->scen_buf);
     if (inst->out_buf) (void)hipFree(inst->out_buf);
     if (inst->mean_buf) (void)hipFree(inst->mean_buf);
@@ -671,17 +813,37 @@ static bool slot_live(int s, uint64_t mk) {
     return (mk & B(s)) != 0;
 }
 
-static uint64_t compute_set(uint64_t used, uint64_t mk) {
+// Slots whose value the rule reads (plus the sources of the Z slots it
+// reads), and the fused computations that produce them.
+static void compute_sets(uint64_t used, uint64_t mk, uint64_t *comp_out, uint64_t *fl_out) {
     uint64_t c = 0;
     for (int s = 0; s < SIGP_N_TERMS; ++s)
         if ((used & B(s)) && slot_live(s, mk)) c |= B(s);
     if (c & B(32)) c |= B(14);
     if (c & B(33)) c |= B(11);
     if (c & B(34)) c |= B(30);
-    if (c & (B(10) | B(28) | B(29))) c |= kNeedNs;
-    if (c & (B(17) | B(18) | B(19) | B(20) | B(28) | B(29))) c |= kNeedStruct;
-    if (c & (B(25) | B(26) | B(51) | B(52))) c |= kNeedPre;
-    return c;
+    uint64_t f = 0;
+    if (c & (B(10) | B(28) | B(29) | B(45))) f |= F_NS;
+    if (c & (B(11) | B(43))) f |= F_REGRET;
+    if (c & (B(12) | B(45) | B(46))) f |= F_NSROB;
+    if (c & B(13)) f |= F_MAXN;
+    if (c & B(14)) f |= F_REACH2;
+    if (c & B(15)) f |= F_NSTW;
+    if (c & (B(17) | B(18) | B(19) | B(20) | B(28) | B(29))) f |= F_STRUCT;
+    if (c & (B(21) | B(22) | B(23))) {
+        const uint64_t k = (c & B(23)) ? 5 : ((c & B(22)) ? 3 : 1);
+        f |= F_REACHK | (k << 40);
+    }
+    if (c & B(30)) f |= F_ATRISK | F_CNOW;
+    if (c & (B(41) | B(43))) f |= F_R2ROB;
+    if (c & (B(47) | B(48))) {
+        const uint64_t k = (c & B(48)) ? 5 : 3;
+        f |= F_REACHKR | (k << 43);
+    }
+    if (c & B(51)) f |= F_FSCORE;
+    if (c & B(52)) f |= F_FNOWROB;
+    *comp_out = c;
+    *fl_out = f;
 }
 
 extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *rules, int32_t T,
@@ -702,7 +864,7 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
         return fail(SIGP_INVALID_ARG, "mask row (n_rules-1)/mask_group out of range");
 
     std::vector<int32_t> ops, args, offs(P), lens(P);
-    std::vector<uint64_t> comp(P), mb(P);
+    std::vector<uint64_t> comp(P), flg(P);
     int max_depth = 1;
     for (int p = 0; p < P; ++p) {
         const int64_t o = rules->offsets[p], L = rules->lengths[p];
@@ -738,7 +900,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
         if (max_depth > SIGP_MAX_STACK)
             return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack deeper than " +
                                               std::to_string(SIGP_MAX_STACK));
-        const uint64_t c = compute_set(used, mk);
+        uint64_t c = 0, fb = 0;
+        compute_sets(used, mk, &c, &fb);
         offs[p] = (int32_t)ops.size();
         lens[p] = (int32_t)L;
         for (int64_t k = 0; k < L; ++k) {
@@ -750,19 +913,54 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
             args.push_back(a1);
         }
         comp[p] = c;
-        mb[p] = mk;
+        flg[p] = fb;
+    }
+    // Longest-first launch order of the (rule, scenario block) items: a rule's
+    // relative cost per candidate from the scans its flags select (the
+    // per-neighbour weights were measured on MI355X), so the expensive rules
+    // do not form the tail of the launch.
+    const int nblk = (T + 63) / 64;
+    std::vector<int32_t> order;
+    if (P > 0 && nblk > 0) {
+        std::vector<float> cost(P);
+        for (int p = 0; p < P; ++p) {
+            const uint64_t f = flg[p];
+            float c = 1.0f + 0.05f * lens[p];
+            if (f & (F_NS | F_MAXN | F_REACH2)) c += 1.0f;
+            if (f & F_REACH2) c += 1.0f;
+            if (f & (F_NSROB | F_R2ROB | F_REACHKR)) c += 1.0f;
+            if (f & F_R2ROB) c += 1.0f;
+            if (f & F_REGRET) c += 0.5f;
+            if (f & F_STRUCT) c += 2.0f;
+            if (f & F_NSTW) c += 1.0f;
+            if (f & F_ATRISK) c += 8.0f;
+            if (f & F_REACHK) c += 1.5f * (float)((f >> 40) & 7);
+            if (f & F_REACHKR) c += 1.5f * (float)((f >> 43) & 7);
+            if (comp[p] & (B(44) | B(49) | B(50) | B(53) | B(54) | B(55) | B(56) | B(57) | B(59) | B(60)))
+                c += 4.0f;
+            cost[p] = c;
+        }
+        order.resize((size_t)P * nblk);
+        for (size_t k = 0; k < order.size(); ++k) order[k] = (int32_t)k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int32_t a, int32_t b) { return cost[a / nblk] > cost[b / nblk]; });
     }
     sigp_batch *bt = new sigp_batch();
     bt->inst = inst;
     bt->P = P;
     bt->T = T;
     bt->group_size = group_size;
-    bt->stk = max_depth <= 16 ? 16 : 32;
+    bt->stk = max_depth <= 8 ? 8 : (max_depth <= 16 ? 16 : 32);
+    if (const char *e = std::getenv("SIGP_WAVES_PER_ITEM")) {
+        const int w = std::atoi(e);
+        if (w == 1 || w == 2 || w == 4) bt->waves_per_item = w;
+    }
     bt->rows_needed = P > 0 ? (int64_t)((P - 1) / group_size + 1) * T : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes + 8); return o; };
     const size_t o_ops = take(ops.size() * 4), o_args = take(args.size() * 4), o_off = take(P * 4),
-                 o_len = take(P * 4), o_comp = take(P * 8), o_mb = take(P * 8);
+                 o_len = take(P * 4), o_comp = take(P * 8), o_fl = take(P * 8),
+                 o_ord = take(order.size() * 4);
     char *base = nullptr;
     hipError_t e = hipMalloc((void **)&base, off);
     if (e != hipSuccess) { delete bt; return fail(SIGP_OOM, "batch allocation failed"); }
@@ -773,7 +971,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     if (up(o_ops, ops.data(), ops.size() * 4) != hipSuccess ||
         up(o_args, args.data(), args.size() * 4) != hipSuccess ||
         up(o_off, offs.data(), P * 4) != hipSuccess || up(o_len, lens.data(), P * 4) != hipSuccess ||
-        up(o_comp, comp.data(), P * 8) != hipSuccess || up(o_mb, mb.data(), P * 8) != hipSuccess) {
+        up(o_comp, comp.data(), P * 8) != hipSuccess || up(o_fl, flg.data(), P * 8) != hipSuccess ||
+        up(o_ord, order.data(), order.size() * 4) != hipSuccess) {
         (void)hipFree(base);
         delete bt;
         return fail(SIGP_DEVICE_ERROR, "batch upload failed");
@@ -783,7 +982,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     bt->rs.off = (cip)(base + o_off);
     bt->rs.len = (cip)(base + o_len);
     bt->rs.comp = (cup)(base + o_comp);
-    bt->rs.mbits = (cup)(base + o_mb);
+    bt->rs.flags = (cup)(base + o_fl);
+    bt->rs.order = order.empty() ? nullptr : (cip)(base + o_ord);
     bt->rs.P = P;
     *out = bt;
     return SIGP_OK;
@@ -812,9 +1012,8 @@ static void launch_sim(const sigp_batch *bt, const double *sampled, double *out,
     S.T = bt->T;
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
-    const long long waves = (long long)bt->P * S.nblk;
-    const unsigned grid = (unsigned)((waves + 3) / 4);
-    sim_kernel<NW, STK><<<grid, 256, 0, stream>>>(I->dev, bt->rs, S, out);
+    const unsigned items = (unsigned)((long long)bt->P * S.nblk);
+    sim_kernel<NW, STK><<<items, 64 * bt->waves_per_item, 0, stream>>>(I->dev, I->tabs, bt->rs, S, out);
 }
 
 extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,
@@ -837,7 +1036,11 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
         pairs = bt->pairs_tmp;
     }
     const int nwb = nw_bucket(bt->inst->n);
-    if (bt->stk == 16) {
+    if (bt->stk == 8) {
+        if (nwb == 4) launch_sim<4, 8>(bt, sampled_dev, pairs, s);
+        else if (nwb == 9) launch_sim<9, 8>(bt, sampled_dev, pairs, s);
+        else launch_sim<16, 8>(bt, sampled_dev, pairs, s);
+    } else if (bt->stk == 16) {
         if (nwb == 4) launch_sim<4, 16>(bt, sampled_dev, pairs, s);
         else if (nwb == 9) launch_sim<9, 16>(bt, sampled_dev, pairs, s);
         else launch_sim<16, 16>(bt, sampled_dev, pairs, s);
