@@ -127,6 +127,9 @@ def main():
     def step():
         batch.run(sampled.data_ptr(), T, pairs.data_ptr(), means.data_ptr(), stream.cuda_stream)
 
+    waves = batch.autotune(sampled.data_ptr(), T, stream.cuda_stream)
+    log("autotuned mapping: %d wavefronts per (rule, 64-scenario block)" % waves)
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -169,7 +172,8 @@ def main():
                                "(ramped half-and-half, depth 2-6 levels, EXTRA-18 terminals) x %d "
                                "Gamma(4, mean 10) scenarios" % (P, T),
                    "rules": P, "scenarios": T, "n": wl.inst.n, "m": wl.inst.m,
-                   "parallelism": "1 GPU, wave = rule x 64 scenarios"},
+                   "parallelism": "1 GPU; workgroup = rule x 64 scenarios, %d cooperating "
+                                  "wavefronts (autotuned)" % waves},
         "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:

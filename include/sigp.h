@@ -67,8 +67,7 @@ typedef struct sigp_batch sigp_batch;
 /* Text of the last error on the calling thread ("" if none). */
 const char *sigp_last_error(void);
 
-/* Library/device information: device name into buf, CU count and the FP64
- * lane-op peak (ops/s) the roofline uses. */
+/* Device information: name into name_buf, compute-unit count and clock (GHz). */
 sigp_status sigp_device_info(char *name_buf, size_t buf_len, int32_t *n_cu, double *clock_ghz);
 
 /* Replaces FactoryContext::FactoryContext (engine/factory.cuh:129-137):
@@ -112,6 +111,28 @@ sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *rules, int3
 sigp_status sigp_batch_run(sigp_batch *batch, const double *sampled_dev, int64_t n_rows,
                            double *out_pairs_dev, double *out_mean_dev, void *stream);
 sigp_status sigp_batch_destroy(sigp_batch *batch);
+
+/* Launch mapping of the simulation kernel (no counterpart in the reference,
+ * whose kernels always run one thread per (rule, scenario) pair).  A
+ * workgroup simulates one rule on a block of 64 scenarios (one scenario per
+ * lane); waves_per_item wavefronts (1, 2 or 4) split each decision's
+ * candidates between them.  Every mapping gives bit-identical results. */
+typedef struct {
+    int32_t waves_per_item;   /* 1, 2 or 4; 0 = library default */
+} sigp_launch_config;
+
+/* Mapping of one batch. */
+sigp_status sigp_batch_configure(sigp_batch *batch, const sigp_launch_config *cfg);
+
+/* Runs the batch once per candidate mapping on `stream` (device pointers as
+ * in sigp_batch_run), keeps the fastest, and reports it in cfg_out
+ * (nullable).  Synchronous. */
+sigp_status sigp_batch_autotune(sigp_batch *batch, const double *sampled_dev, int64_t n_rows,
+                                void *stream, sigp_launch_config *cfg_out);
+
+/* Default mapping for batches created later on this instance, including the
+ * ones sigp_evaluate creates internally. */
+sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_launch_config *cfg);
 
 /* Number of rules and (rule, scenario) pairs in a batch. */
 sigp_status sigp_batch_shape(const sigp_batch *batch, int32_t *n_rules, int64_t *n_pairs);

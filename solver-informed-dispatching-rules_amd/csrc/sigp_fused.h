@@ -66,6 +66,12 @@ __device__ __forceinline__ D8 ld8(cdp row, int j0) {
     return r;
 }
 
+// True when every active lane holds the same value of v.
+__device__ __forceinline__ bool wave_uniform(int v) {
+    const int v0 = __builtin_amdgcn_readfirstlane(v);
+    return __builtin_amdgcn_ballot_w64(v != v0) == 0;
+}
+
 // The eight bits j0..j0+7 of a per-lane bit set (j0 a multiple of 8).
 template <int NW>
 __device__ __forceinline__ uint32_t bits8(const uint32_t (&a)[NW], int j0) {
@@ -291,8 +297,26 @@ template <int NW>
 __device__ double reach2_leg2(const DevInst &I, const FusedTabs &F, int i, int bj, double fj,
                               const uint32_t (&vis)[NW], bool rob) {
     const int pj = bj + 1;
-    const cdp tcol = rob ? F.t2nc : F.t0nc;
     double bv2 = 0.0, s2 = 0.0;
+    if (wave_uniform(pj)) {
+        // every lane continues from the same neighbour: its row, read as scalars
+        const int pu = __builtin_amdgcn_readfirstlane(pj), bu = pu - 1;
+        const cdp tr = (rob ? F.t2n : F.t0n) + (size_t)pu * F.ldp, rr = F.r + (size_t)pu * F.ldp;
+        for (int k0 = 0; k0 < I.n; k0 += 8) {
+            const uint32_t vb = bits8<NW>(vis, k0);
+            const D8 T = ld8(tr, k0), R = ld8(rr, k0), S = ld8(F.score, k0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = k0 + u;
+                const bool take = k != i && k != bu && !((vb >> u) & 1u) && fj <= T.v[u] &&
+                                  R.v[u] > bv2;
+                bv2 = sel(take, R.v[u], bv2);
+                s2 = sel(take, S.v[u], s2);
+            }
+        }
+        return I.score[bu] + s2;
+    }
+    const cdp tcol = rob ? F.t2nc : F.t0nc;
     for (int k0 = 0; k0 < I.n; k0 += 4) {
         const uint32_t vb = (wsel<NW>(vis, k0 >> 5) >> (k0 & 31)) & 0xfu;
         double T[4], R[4];
@@ -333,7 +357,24 @@ __device__ void reach_path(const DevInst &I, const FusedTabs &F, int c, int best
     double t = t0;
     int best = best1;
     for (int step = 0; step < K; ++step) {
-        if (step > 0) {
+        if (step > 0 && wave_uniform(cur)) {
+            // every lane stands at the same waypoint: its row, read as scalars
+            const int cu = __builtin_amdgcn_readfirstlane(cur);
+            const cdp tr = (rob ? F.t2 : F.t0) + (size_t)cu * F.ldp, rr = F.r + (size_t)cu * F.ldp;
+            best = -1;
+            double bv = -1.0;
+            for (int q0 = 0; q0 < I.n; q0 += 8) {
+                const uint32_t sb = bits8<NW>(scr, q0);
+                const D8 T = ld8(tr, q0), R = ld8(rr, q0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const bool take = !((sb >> u) & 1u) && t <= T.v[u] && R.v[u] > bv;
+                    bv = sel(take, R.v[u], bv);
+                    best = take ? q0 + u : best;
+                }
+            }
+            if (best < 0) break;
+        } else if (step > 0) {
             best = -1;
             double bv = -1.0;
             for (int q0 = 0; q0 < I.n; q0 += 4) {

@@ -543,7 +543,13 @@ struct sigp_instance {
     size_t out_cap = 0;
     double *mean_buf = nullptr;
     size_t mean_cap = 0;
+    int waves_per_item = 0;         // 0 = choose per batch
 };
+
+// Two cooperating wavefronts per block once there are enough blocks to fill
+// the GPU twice over (256 CUs x 8 resident waves); four below that, so a
+// small batch still spreads its candidates over more of the chip.
+static int default_waves(long long items) { return items >= 2048 ? 2 : 4; }
 
 struct sigp_batch {
     sigp_instance *inst = nullptr;
@@ -951,10 +957,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     bt->T = T;
     bt->group_size = group_size;
     bt->stk = max_depth <= 8 ? 8 : (max_depth <= 16 ? 16 : 32);
-    if (const char *e = std::getenv("SIGP_WAVES_PER_ITEM")) {
-        const int w = std::atoi(e);
-        if (w == 1 || w == 2 || w == 4) bt->waves_per_item = w;
-    }
+    bt->waves_per_item = inst->waves_per_item > 0 ? inst->waves_per_item
+                                                   : default_waves((long long)P * ((T + 63) / 64));
     bt->rows_needed = P > 0 ? (int64_t)((P - 1) / group_size + 1) * T : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes + 8); return o; };
@@ -994,6 +998,24 @@ extern "C" sigp_status sigp_batch_destroy(sigp_batch *bt) {
     if (bt->dbuf) (void)hipFree(bt->dbuf);
     if (bt->pairs_tmp) (void)hipFree(bt->pairs_tmp);
     delete bt;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_batch_configure(sigp_batch *bt, const sigp_launch_config *cfg) {
+    if (!bt || !cfg) return fail(SIGP_INVALID_ARG, "batch and cfg must be non-NULL");
+    const int w = cfg->waves_per_item;
+    if (w == 0) bt->waves_per_item = default_waves((long long)bt->P * ((bt->T + 63) / 64));
+    else if (w == 1 || w == 2 || w == 4) bt->waves_per_item = w;
+    else return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_launch_config *cfg) {
+    if (!inst || !cfg) return fail(SIGP_INVALID_ARG, "inst and cfg must be non-NULL");
+    const int w = cfg->waves_per_item;
+    if (w != 0 && w != 1 && w != 2 && w != 4)
+        return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
+    inst->waves_per_item = w;
     return SIGP_OK;
 }
 
@@ -1055,6 +1077,40 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
         HIP_TRY(hipGetLastError());
     }
     return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_batch_autotune(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,
+                                           void *stream, sigp_launch_config *cfg_out) {
+    if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
+    if (bt->P == 0 || bt->T == 0) {
+        if (cfg_out) cfg_out->waves_per_item = bt->waves_per_item;
+        return SIGP_OK;
+    }
+    if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->P * bt->T * 8));
+    hipStream_t s = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int cand[3] = {1, 2, 4};
+    int best_w = bt->waves_per_item;
+    float best_ms = 0.0f;
+    sigp_status st = SIGP_OK;
+    for (int k = 0; k < 3 && st == SIGP_OK; ++k) {
+        bt->waves_per_item = cand[k];
+        (void)hipEventRecord(e0, s);
+        st = sigp_batch_run(bt, sampled_dev, n_rows, bt->pairs_tmp, nullptr, stream);
+        (void)hipEventRecord(e1, s);
+        if (st != SIGP_OK) break;
+        if (hipEventSynchronize(e1) != hipSuccess) { st = fail(SIGP_DEVICE_ERROR, "autotune run failed"); break; }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (k == 0 || ms < best_ms) { best_ms = ms; best_w = cand[k]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    bt->waves_per_item = best_w;
+    if (cfg_out) cfg_out->waves_per_item = best_w;
+    return st;
 }
 
 static sigp_status ensure(double **buf, size_t *cap, size_t bytes) {

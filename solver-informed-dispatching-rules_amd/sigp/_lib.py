@@ -8,7 +8,7 @@ import ctypes as C
 import os
 from typing import Optional
 
-__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "EXPORTED"]
+__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "EXPORTED"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libsigp.so")
@@ -16,7 +16,8 @@ lib_path = os.path.join(_HERE, "libsigp.so")
 # Every symbol include/sigp.h declares.
 EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp_instance_destroy",
             "sigp_zstats", "sigp_evaluate", "sigp_batch_create", "sigp_batch_run",
-            "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math"]
+            "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math", "sigp_batch_configure",
+            "sigp_batch_autotune", "sigp_instance_configure"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
            4: "SIGP_NO_DEVICE"}
@@ -35,6 +36,10 @@ class InstanceDesc(C.Structure):
                 ("zp", C.c_void_p)]
 
 
+class LaunchConfig(C.Structure):
+    _fields_ = [("waves_per_item", C.c_int32)]
+
+
 class Rules(C.Structure):
     _fields_ = [("ops", C.c_void_p), ("args", C.c_void_p), ("offsets", C.c_void_p),
                 ("lengths", C.c_void_p), ("n_rules", C.c_int32), ("n_nodes", C.c_int64)]
@@ -51,6 +56,13 @@ def lib() -> C.CDLL:
     if not os.path.exists(lib_path):
         raise RuntimeError("libsigp.so not found at %s: run `python -c \"import __graft_entry__ as g; "
                            "g.build()\"` (hipcc, gfx950) first" % lib_path)
+    # A process holds one HIP runtime (soname libamdhip64.so.7).  PyTorch ships
+    # its own build of it and fails to find the GPU on another one, so when
+    # torch is installed it is loaded first and libsigp binds to the same runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(lib_path)
     vp, i32, i64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_double
     sig = {
@@ -65,6 +77,9 @@ def lib() -> C.CDLL:
         "sigp_batch_destroy": ([vp], C.c_int),
         "sigp_batch_shape": ([vp, vp, vp], C.c_int),
         "sigp_device_math": ([i32, vp, vp, i64, vp], C.c_int),
+        "sigp_batch_configure": ([vp, vp], C.c_int),
+        "sigp_batch_autotune": ([vp, vp, i64, vp, vp], C.c_int),
+        "sigp_instance_configure": ([vp, vp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(L, name)

@@ -10,7 +10,7 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import InstanceDesc, Rules, check, lib
+from ._lib import InstanceDesc, LaunchConfig, Rules, check, lib
 from .instance import Instance
 from .rules import N_TERMS, Packed
 
@@ -100,6 +100,18 @@ class Batch:
                                    None if out_mean_ptr is None else C.c_void_p(out_mean_ptr),
                                    None if stream is None else C.c_void_p(stream)))
 
+    def configure(self, waves_per_item: int) -> None:
+        cfg = LaunchConfig(waves_per_item)
+        check(lib().sigp_batch_configure(self._h, C.byref(cfg)))
+
+    def autotune(self, sampled_ptr: int, n_rows: int, stream: Optional[int] = None) -> int:
+        """Time every mapping, keep the fastest; returns its waves_per_item."""
+        cfg = LaunchConfig(0)
+        check(lib().sigp_batch_autotune(self._h, C.c_void_p(sampled_ptr), n_rows,
+                                        None if stream is None else C.c_void_p(stream),
+                                        C.byref(cfg)))
+        return cfg.waves_per_item
+
     def close(self) -> None:
         if self._h:
             lib().sigp_batch_destroy(self._h)
@@ -122,6 +134,11 @@ class FactoryContext:
         d = _desc(inst, self.zp, keep)
         self._h = C.c_void_p()
         check(lib().sigp_instance_create(C.byref(d), C.byref(self._h)))
+
+    def configure(self, waves_per_item: int) -> None:
+        """Default mapping of later batches and evaluate* calls (0 = automatic)."""
+        cfg = LaunchConfig(waves_per_item)
+        check(lib().sigp_instance_configure(self._h, C.byref(cfg)))
 
     def close(self) -> None:
         if self._h:
