@@ -9,11 +9,12 @@ are resident in HBM before timing; each step is one launch of the
 simulation kernel (plus the per-rule mean kernel) on torch's current stream,
 timed with HIP events on that stream.
 
-Prints one JSON line (the contract in the repository README of the driver):
-value = rule x scenario simulations per second on the one GPU, with a
-`roofline` object (FP64 operations of the simulation, counted by the CPU
-oracle's counting build on a sample of pairs, per kernel-second, against the
-MI355X FP64 vector peak) and a `cpu_baseline` object (the reference's own
+Prints one JSON line: value = rule x scenario simulations per second on the
+one GPU, with a `roofline` object (the workload's algorithmic FP64
+operations per simulation, from the committed work model
+tests/golden/work_plain_small.json, times the simulations of one launch,
+divided by the simulation kernel's own event-timed duration, against the
+MI355X FP64 peak) and a `cpu_baseline` object (the reference's own
 simulator, compiled from its sources, timed on the host cores on a bounded
 sample of the same workload).
 """
@@ -28,35 +29,33 @@ sys.path.insert(0, os.path.join(ROOT, "solver-informed-dispatching-rules_amd"))
 
 import numpy as np  # noqa: E402
 
-FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 (vector and matrix) peak
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 (vector and matrix) peak, FMA = 2 flops
+
+# HBM bytes per simulation-kernel launch, keyed by (rules, scenarios): the
+# FETCH_SIZE and WRITE_SIZE counters of profiles/sim_kernel_plain_small_hbm_traffic.md,
+# each scaled by what it reads for a plain copy of known size (x1.9997, x1.0).
+TRAFFIC_BYTES = {(1024, 500): 20963172}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def need_rows(pk, mask):
-    """Per-rule slots the evaluator computes (the library's pruning rule:
-    only slots the rule reads and the mask makes live, plus dependencies)."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import need_rows_for
-    return need_rows_for(pk, mask)
-
-
-def op_count_per_sim(wl, n_rules=24, n_scen=6):
-    """Average FP64 operations per (rule, scenario) simulation."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle
-    from sigp.rules import pack_trees
-    O = Oracle(counting=True)
-    step = max(1, wl.packed.n_rules // n_rules)
-    trees = [wl.trees[i] for i in range(0, wl.packed.n_rules, step)][:n_rules]
-    pk = pack_trees(trees)
-    S = wl.sampled[:: max(1, wl.n_scen // n_scen)][:n_scen]
-    zp = O.zstats(wl.inst)
-    O.reset_ops()
-    O.evaluate(wl.inst, pk, S, S.shape[0], wl.mask, zp, need_rows=need_rows(pk, wl.mask), threads=1)
-    return O.ops() / (pk.n_rules * S.shape[0]), pk.n_rules * S.shape[0]
+def work_model(wl):
+    """Algorithmic FP64 operations per (rule, scenario) simulation of this
+    workload: the committed count of tests/golden/work_plain_small.json (made by
+    tests/golden/make_work_model.py with the counting oracle), used only when
+    the workload's arrays match the ones it was counted on."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_work_model as mwm
+    if not os.path.exists(mwm.OUT):
+        return None
+    with open(mwm.OUT) as f:
+        doc = json.load(f)
+    if doc["digest"] != mwm.workload_digest(wl):
+        log("work model does not describe this workload: roofline omitted")
+        return None
+    return doc
 
 
 def cpu_baseline(wl, zp, target_s=15.0):
@@ -100,6 +99,10 @@ def main():
     ap.add_argument("--scenarios", type=int, default=500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="wavefronts per (rule, scenario block): 1, 2 or 4; 0 = autotune")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after timing, run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -127,8 +130,12 @@ def main():
     def step():
         batch.run(sampled.data_ptr(), T, pairs.data_ptr(), means.data_ptr(), stream.cuda_stream)
 
-    waves = batch.autotune(sampled.data_ptr(), T, stream.cuda_stream)
-    log("autotuned mapping: %d wavefronts per (rule, 64-scenario block)" % waves)
+    if args.waves:
+        batch.configure(args.waves)
+        waves, how = args.waves, "fixed"
+    else:
+        waves, how = batch.autotune(sampled.data_ptr(), T, stream.cuda_stream), "autotuned"
+    log("%s mapping: %d wavefronts per (rule, 64-scenario block)" % (how, waves))
 
     for _ in range(args.warmup):
         step()
@@ -153,12 +160,35 @@ def main():
     assert np.all(np.isfinite(pr)) and np.all(pr == np.round(pr))
 
     roof = None
-    if not args.no_roofline:
-        ops_per_sim, sample = op_count_per_sim(wl)
-        achieved = ops_per_sim * sims / (ms_step / 1000.0) / 1e12
+    model = None if args.no_roofline else work_model(wl)
+    if model is not None:
+        # The simulation kernel alone (no per-rule mean kernel), same stream.
+        kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(args.steps)]
+        for a, b in kev:
+            a.record(stream)
+            batch.run(sampled.data_ptr(), T, pairs.data_ptr(), None, stream.cuda_stream)
+            b.record(stream)
+        torch.cuda.synchronize()
+        k_ms = float(np.mean([a.elapsed_time(b) for a, b in kev]))
+        log("simulation kernel ms:", "%.3f" % k_ms)
+        ops_per_sim = model["ops_per_sim"]
+        achieved = ops_per_sim * sims / (k_ms / 1000.0) / 1e12
         roof = {"bound": "valu-fp64", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
-                "ops_per_sim": round(ops_per_sim, 1), "ops_sample_sims": sample}
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+                "traffic": TRAFFIC_BYTES.get((P, T)),
+                "kernel_ms": round(k_ms, 3), "ops_per_sim": round(ops_per_sim, 1),
+                "issue_frac": round(achieved / (FP64_PEAK_TFLOPS / 2), 5)}
+    if args.copy_probe_mib > 0:
+        # One plain device-to-device copy of known size, so that a PMC run of
+        # this command can scale FETCH_SIZE / WRITE_SIZE to bytes.
+        nb = args.copy_probe_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy probe: %d bytes read, %d bytes written" % (nb, nb))
+        del src, dst
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(wl, ctx.zp)
@@ -173,7 +203,7 @@ def main():
                                "Gamma(4, mean 10) scenarios" % (P, T),
                    "rules": P, "scenarios": T, "n": wl.inst.n, "m": wl.inst.m,
                    "parallelism": "1 GPU; workgroup = rule x 64 scenarios, %d cooperating "
-                                  "wavefronts (autotuned)" % waves},
+                                  "wavefronts (%s)" % (waves, how)},
         "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:

@@ -116,7 +116,11 @@ sigp_status sigp_batch_destroy(sigp_batch *batch);
  * whose kernels always run one thread per (rule, scenario) pair).  A
  * workgroup simulates one rule on a block of 64 scenarios (one scenario per
  * lane); waves_per_item wavefronts (1, 2 or 4) split each decision's
- * candidates between them.  Every mapping gives bit-identical results. */
+ * candidates between them.  Every mapping gives bit-identical results.
+ * Each wavefront keeps the candidate's terminal values and the postfix stack
+ * in LDS, (distinct slots read + deepest stack - 1) * 512 bytes, and a
+ * workgroup stays within 64 KiB of LDS: a batch whose rules need more runs
+ * with fewer wavefronts than requested (autotune skips such mappings). */
 typedef struct {
     int32_t waves_per_item;   /* 1, 2 or 4; 0 = library default */
 } sigp_launch_config;

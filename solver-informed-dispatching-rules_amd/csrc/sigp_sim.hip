@@ -221,10 +221,12 @@ __global__ void zreduce_kernel(int n, const double *__restrict__ feat, const int
 
 struct RuleSet {
     cip ops, args, off, len;
-    cup comp;    // per rule: slots whose value the rule reads
+    cup rd;      // per rule: slots the rule reads (its leaves index them compactly)
     cup flags;   // per rule: fused computations (F_*) and REACH path lengths
     cip order;   // launch order of (rule, scenario block) items, or null
     int P;
+    int KL;      // value-store rows per wavefront: max over rules of popcount(rd), >= 1
+    int SL;      // stack rows per wavefront: max postfix depth - 1, >= 1
 };
 
 struct ScenSpec {
@@ -234,21 +236,28 @@ struct ScenSpec {
     int nblk;                // ceil(T / 64)
 };
 
-// Postfix interpreter (engine/sim_core.hpp:11-41) with a uniform stack pointer.
-template <int STK>
-__device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L, const double *T) {
-    double stk[STK];
-    int sp = 0;
+// Postfix interpreter (engine/sim_core.hpp:11-41).  Leaves read the
+// candidate's terminal values from the wave's LDS value store (`tv`, one
+// 64-lane row per compact slot index); the stack lives in LDS below them
+// (`st`, one row per level) with its top element in a register.  The stack
+// depth is wave-uniform, so every LDS access is one conflict-free row.
+__device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L,
+                                            const double *tv, double *st) {
+    double top = 0.0;
+    int d = 0;   // values on the stack; the topmost is `top`
     for (int k = 0; k < L; ++k) {
         const int o = ops[k];
-        if (o == -1) {
-            stk[sp++] = T[args[2 * k]];
-        } else if (o == -2) {
-            stk[sp++] = args[2 * k] / 1000.0;
+        if (o < 0) {
+            double v;
+            if (o == -1) v = tv[args[2 * k] * 64];
+            else v = args[2 * k] / 1000.0;
+            if (d > 0) st[(d - 1) * 64] = top;
+            top = v;
+            ++d;
         } else {
-            const double b = stk[sp - 1];
-            const double a = stk[sp - 2];
-            sp -= 2;
+            const double b = top;
+            const double a = st[(d - 2) * 64];
+            --d;
             double r;
             if (o == 0) r = a + b;
             else if (o == 1) r = a - b;
@@ -263,20 +272,24 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
                 else if (e > 1.0) e = 1.0;
                 r = sigp_pow_dd(base, 1.0 + e);
             }
-            stk[sp++] = r;
+            top = r;
         }
     }
-    return stk[0];
+    return top;
 }
 
 // One decision of pick_best (engine/sim_core.hpp:43-204) for every running lane.
 // Candidates i0, i0 + istep, ... of one decision; returns this subset's
 // first maximum (index, priority) -- the reference's strict ">" scan in index
 // order -- so that subsets evaluated by cooperating wavefronts merge exactly.
-template <int NW, int STK>
+// `rd` holds the slots the rule reads; slot s is stored at compact index
+// popcount(rd & (bit(s) - 1)) of the value store, as its leaves were
+// renumbered when the batch was built.
+template <int NW>
 __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
-                      uint64_t comp, uint64_t fl, int cur, double ct, int rem,
-                      const uint32_t (&vis)[NW], int i0, int istep, double &best_pri_out) {
+                      uint64_t rd, uint64_t fl, int cur, double ct, int rem,
+                      const uint32_t (&vis)[NW], int i0, int istep, double *tv, double *st,
+                      double &best_pri_out) {
     // Per-decision feasibility from the lane's current POI: fnow (the
     // candidate test, :78-82), cnow (close time only, ATRISK :239-240) and
     // the robust bits of NFEASROB (:68); thresholds read down column `cur`.
@@ -326,7 +339,6 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
 
     int best = -1;
     double best_pri = -1e300;
-    double T[SIGP_N_TERMS];
     for (int i = i0; i < I.n; i += istep) {
         const uint32_t b = 1u << (i & 31);
         const int w = i >> 5;
@@ -339,16 +351,17 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
         const double tc = I.close[i] - ct;
         const double tsv = to > ta ? to : ta;
         const double tfv = tsv + di;
-        T[0] = I.score[i];
-        T[1] = di;
-        T[2] = to;
-        T[3] = tc;
-        T[4] = ta;
-        T[5] = I.d2d[i];
-        T[6] = tsv;
-        T[7] = tfv;
-        T[8] = tc - ta;
-        T[9] = rem * I.tmax + (I.tmax - ct);
+#define PUT(sl, val) tv[__builtin_popcountll(rd & (B(sl) - 1)) * 64] = (val)
+        if (rd & B(0)) PUT(0, I.score[i]);
+        if (rd & B(1)) PUT(1, di);
+        if (rd & B(2)) PUT(2, to);
+        if (rd & B(3)) PUT(3, tc);
+        if (rd & B(4)) PUT(4, ta);
+        if (rd & B(5)) PUT(5, I.d2d[i]);
+        if (rd & B(6)) PUT(6, tsv);
+        if (rd & B(7)) PUT(7, tfv);
+        if (rd & B(8)) PUT(8, tc - ta);
+        if (rd & B(9)) PUT(9, rem * I.tmax + (I.tmax - ct));
         const double fin = ct + tfv;
         const double arr = ct + ta;
         const double st_c = arr > oi ? arr : oi;
@@ -356,67 +369,68 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
 
         CandVals v{};
         if (fl & 0x7FFull) cand_scans<NW>(I, F, fl, kpath, kpathr, i, fin, st_c, fin_c, vis, fnow, cnow, v);
-        if (comp & B(10)) T[10] = v.ns;
-        if (comp & B(11)) T[11] = v.regret;
-        if (comp & B(12)) T[12] = v.nsrob;
-        if (comp & B(13)) T[13] = v.maxn;
-        if (comp & B(14)) T[14] = v.reach2;
-        if (comp & B(15)) T[15] = v.nstw;
-        if (comp & B(16)) T[16] = t_rand(i);
-        if (comp & B(17)) T[17] = v.fcnt;
-        if (comp & B(18)) T[18] = v.fscore;
-        if (comp & B(19)) T[19] = v.dslack;
-        if (comp & B(20)) T[20] = v.pbust;
-        if (comp & B(21)) T[21] = v.rk1;
-        if (comp & B(22)) T[22] = v.rk3;
-        if (comp & B(23)) T[23] = v.rk5;
-        if (comp & B(24)) T[24] = sat;
-        if (comp & B(25)) T[25] = vfrac;
-        if (comp & B(26)) T[26] = nfeas;
-        if (comp & B(27)) T[27] = (double)rem;
-        if (comp & B(28)) T[28] = v.fscore / (v.ns + 1.0);
-        if (comp & B(29)) T[29] = v.fscore - v.ns;
-        if (comp & B(30)) T[30] = v.eloss;
-        if (comp & B(31)) T[31] = (T[3] - I.zp[0]) / I.zp[5];
-        if (comp & B(32)) T[32] = (T[14] - I.zp[1]) / I.zp[6];
-        if (comp & B(33)) T[33] = (T[11] - I.zp[2]) / I.zp[7];
-        if (comp & B(34)) T[34] = (T[30] - I.zp[3]) / I.zp[8];
-        if (comp & B(35)) T[35] = (T[5] - I.zp[4]) / I.zp[9];
-        if (comp & B(36)) T[36] = 0.1;
-        if (comp & B(37)) T[37] = 0.2;
-        if (comp & B(38)) T[38] = 0.3;
-        if (comp & B(39)) T[39] = 0.4;
-        if (comp & B(40)) T[40] = 0.5;
-        if (comp & B(41)) T[41] = v.r2rob;
-        if (comp & B(42)) {
+        if (rd & B(10)) PUT(10, v.ns);
+        if (rd & B(11)) PUT(11, v.regret);
+        if (rd & B(12)) PUT(12, v.nsrob);
+        if (rd & B(13)) PUT(13, v.maxn);
+        if (rd & B(14)) PUT(14, v.reach2);
+        if (rd & B(15)) PUT(15, v.nstw);
+        if (rd & B(16)) PUT(16, t_rand(i));
+        if (rd & B(17)) PUT(17, v.fcnt);
+        if (rd & B(18)) PUT(18, v.fscore);
+        if (rd & B(19)) PUT(19, v.dslack);
+        if (rd & B(20)) PUT(20, v.pbust);
+        if (rd & B(21)) PUT(21, v.rk1);
+        if (rd & B(22)) PUT(22, v.rk3);
+        if (rd & B(23)) PUT(23, v.rk5);
+        if (rd & B(24)) PUT(24, sat);
+        if (rd & B(25)) PUT(25, vfrac);
+        if (rd & B(26)) PUT(26, nfeas);
+        if (rd & B(27)) PUT(27, (double)rem);
+        if (rd & B(28)) PUT(28, v.fscore / (v.ns + 1.0));
+        if (rd & B(29)) PUT(29, v.fscore - v.ns);
+        if (rd & B(30)) PUT(30, v.eloss);
+        if (rd & B(31)) PUT(31, (tc - I.zp[0]) / I.zp[5]);
+        if (rd & B(32)) PUT(32, (v.reach2 - I.zp[1]) / I.zp[6]);
+        if (rd & B(33)) PUT(33, (v.regret - I.zp[2]) / I.zp[7]);
+        if (rd & B(34)) PUT(34, (v.eloss - I.zp[3]) / I.zp[8]);
+        if (rd & B(35)) PUT(35, (I.d2d[i] - I.zp[4]) / I.zp[9]);
+        if (rd & B(36)) PUT(36, 0.1);
+        if (rd & B(37)) PUT(37, 0.2);
+        if (rd & B(38)) PUT(38, 0.3);
+        if (rd & B(39)) PUT(39, 0.4);
+        if (rd & B(40)) PUT(40, 0.5);
+        if (rd & B(41)) PUT(41, v.r2rob);
+        if (rd & B(42)) {
             double den = I.tmax - ct;
             if (den < 1.0) den = 1.0;
-            T[42] = (ta + di) / den;
+            PUT(42, (ta + di) / den);
         }
-        if (comp & B(43)) T[43] = v.r2rob - v.regret;
-        if (comp & B(44)) T[44] = t_skiploss<NW>(I, i, cur, ct, vis);
-        if (comp & B(45)) T[45] = v.ns - v.nsrob;
-        if (comp & B(46)) {
+        if (rd & B(43)) PUT(43, v.r2rob - v.regret);
+        if (rd & B(44)) PUT(44, t_skiploss<NW>(I, i, cur, ct, vis));
+        if (rd & B(45)) PUT(45, v.ns - v.nsrob);
+        if (rd & B(46)) {
             double den = ta + I.d2d[i];
             if (den < 1.0) den = 1.0;
-            T[46] = (I.score[i] + v.nsrob) / den;
+            PUT(46, (I.score[i] + v.nsrob) / den);
         }
-        if (comp & B(47)) T[47] = v.rkr3;
-        if (comp & B(48)) T[48] = v.rkr5;
-        if (comp & B(49)) T[49] = t_ns_alpha<NW>(I, i, fin, vis, false);
-        if (comp & B(50)) T[50] = t_ns_alpha<NW>(I, i, fin, vis, true);
-        if (comp & B(51)) T[51] = fsc;
-        if (comp & B(52)) T[52] = nfr;
-        if (comp & B(53)) T[53] = t_ns_rob_m<NW>(I, i, fin, di, vis, 0.5, true);
-        if (comp & B(54)) T[54] = t_ns_rob_m<NW>(I, i, fin, di, vis, 2.0, true);
-        if (comp & B(55)) T[55] = t_nnfwd<NW>(I, i, fin, vis);
-        if (comp & B(56)) T[56] = t_nnk3<NW>(I, i, fin, vis);
-        if (comp & B(57)) T[57] = t_nnchain<NW>(I, i, st_c, vis, 3);
-        if (comp & B(58)) T[58] = I.distT[cur] + I.d2d[i] - ta;   // dist[cur][0]
-        if (comp & B(59)) T[59] = t_detour<NW>(I, i, cur, fin, ta, vis);
-        if (comp & B(60)) T[60] = t_scorerank<NW>(I, i, fin, vis);
+        if (rd & B(47)) PUT(47, v.rkr3);
+        if (rd & B(48)) PUT(48, v.rkr5);
+        if (rd & B(49)) PUT(49, t_ns_alpha<NW>(I, i, fin, vis, false));
+        if (rd & B(50)) PUT(50, t_ns_alpha<NW>(I, i, fin, vis, true));
+        if (rd & B(51)) PUT(51, fsc);
+        if (rd & B(52)) PUT(52, nfr);
+        if (rd & B(53)) PUT(53, t_ns_rob_m<NW>(I, i, fin, di, vis, 0.5, true));
+        if (rd & B(54)) PUT(54, t_ns_rob_m<NW>(I, i, fin, di, vis, 2.0, true));
+        if (rd & B(55)) PUT(55, t_nnfwd<NW>(I, i, fin, vis));
+        if (rd & B(56)) PUT(56, t_nnk3<NW>(I, i, fin, vis));
+        if (rd & B(57)) PUT(57, t_nnchain<NW>(I, i, st_c, vis, 3));
+        if (rd & B(58)) PUT(58, I.distT[cur] + I.d2d[i] - ta);   // dist[cur][0]
+        if (rd & B(59)) PUT(59, t_detour<NW>(I, i, cur, fin, ta, vis));
+        if (rd & B(60)) PUT(60, t_scorerank<NW>(I, i, fin, vis));
 
-        double pri = eval_rule<STK>(ops, args, L, T);
+#undef PUT
+        double pri = eval_rule(ops, args, L, tv, st);
         if (pri != pri) pri = -1e300;
         if (pri > best_pri) { best_pri = pri; best = i; }
     }
@@ -430,11 +444,14 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
 // decision, and the W partial first-maxima are merged through LDS (largest
 // priority, then lowest index), which is exactly the reference's argmax.  All
 // W wavefronts hold identical route state, so they leave the loop together.
-template <int NW, int STK>
+// Dynamic LDS: per wavefront KL + SL rows of 64 doubles (value store, then
+// stack), so a launch takes W * (KL + SL) * 512 bytes.
+template <int NW>
 __global__ __launch_bounds__(256) void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
                                                   double *__restrict__ out) {
     __shared__ double lpri[4][64];
     __shared__ int lidx[4][64];
+    extern __shared__ double lds_rows[];
     const int item = R.order ? R.order[blockIdx.x] : (int)blockIdx.x;
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -446,10 +463,12 @@ __global__ __launch_bounds__(256) void sim_kernel(DevInst I, FusedTabs F, RuleSe
     const int off = R.off[p], L = R.len[p];
     const cip ops = R.ops + off;
     const cip args = R.args + 2 * (size_t)off;
-    const uint64_t comp = R.comp[p];
+    const uint64_t rd = R.rd[p];
     const uint64_t fl = R.flags[p];
     const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
     const double *__restrict__ samp = S.sampled + row * (size_t)I.n;
+    double *tv = lds_rows + (size_t)wv * (R.KL + R.SL) * 64 + lane;
+    double *st = tv + (size_t)R.KL * 64;
 
     uint32_t vis[NW];
 #pragma unroll
@@ -462,7 +481,7 @@ __global__ __launch_bounds__(256) void sim_kernel(DevInst I, FusedTabs F, RuleSe
         double bp = -1e300;
         if (running) {
             const int rem = I.m - veh - 1;
-            best = decide<NW, STK>(I, F, ops, args, L, comp, fl, cur, ct, rem, vis, wv, W, bp);
+            best = decide<NW>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis, wv, W, tv, st, bp);
         }
         if (W > 1) {
             lpri[wv][lane] = bp;
@@ -553,7 +572,7 @@ static int default_waves(long long items) { return items >= 2048 ? 2 : 4; }
 
 struct sigp_batch {
     sigp_instance *inst = nullptr;
-    int P = 0, T = 0, group_size = 1, stk = 16;
+    int P = 0, T = 0, group_size = 1;
     int waves_per_item = 4;         // cooperating wavefronts per (rule, scenario block)
     int64_t rows_needed = 0;
     void *dbuf = nullptr;
@@ -819,12 +838,14 @@ static bool slot_live(int s, uint64_t mk) {
     return (mk & B(s)) != 0;
 }
 
-// Slots whose value the rule reads (plus the sources of the Z slots it
-// reads), and the fused computations that produce them.
-static void compute_sets(uint64_t used, uint64_t mk, uint64_t *comp_out, uint64_t *fl_out) {
+// Slots whose value the rule reads (rd_out), the same closed under the Z
+// slots' sources (comp_out), and the fused computations that produce them.
+static void compute_sets(uint64_t used, uint64_t mk, uint64_t *rd_out, uint64_t *comp_out,
+                         uint64_t *fl_out) {
     uint64_t c = 0;
     for (int s = 0; s < SIGP_N_TERMS; ++s)
         if ((used & B(s)) && slot_live(s, mk)) c |= B(s);
+    *rd_out = c;
     if (c & B(32)) c |= B(14);
     if (c & B(33)) c |= B(11);
     if (c & B(34)) c |= B(30);
@@ -870,8 +891,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
         return fail(SIGP_INVALID_ARG, "mask row (n_rules-1)/mask_group out of range");
 
     std::vector<int32_t> ops, args, offs(P), lens(P);
-    std::vector<uint64_t> comp(P), flg(P);
-    int max_depth = 1;
+    std::vector<uint64_t> comp(P), rdv(P), flg(P);
+    int max_depth = 1, max_k = 1;
     for (int p = 0; p < P; ++p) {
         const int64_t o = rules->offsets[p], L = rules->lengths[p];
         if (o < 0 || L < 1 || o + L > rules->n_nodes)
@@ -906,19 +927,24 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
         if (max_depth > SIGP_MAX_STACK)
             return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack deeper than " +
                                               std::to_string(SIGP_MAX_STACK));
-        uint64_t c = 0, fb = 0;
-        compute_sets(used, mk, &c, &fb);
+        uint64_t r = 0, c = 0, fb = 0;
+        compute_sets(used, mk, &r, &c, &fb);
+        max_k = std::max(max_k, __builtin_popcountll(r));
         offs[p] = (int32_t)ops.size();
         lens[p] = (int32_t)L;
         for (int64_t k = 0; k < L; ++k) {
             int op = rules->ops[o + k];
             int a0 = rules->args[2 * (o + k)], a1 = rules->args[2 * (o + k) + 1];
-            if (op == -1 && !(c & B(a0))) { op = -2; a0 = 0; a1 = 0; }   // dead slot reads 0.0
+            if (op == -1) {
+                if (!(r & B(a0))) { op = -2; a0 = 0; a1 = 0; }   // dead slot reads 0.0
+                else a0 = __builtin_popcountll(r & (B(a0) - 1));    // compact value-store row
+            }
             ops.push_back(op);
             args.push_back(a0);
             args.push_back(a1);
         }
         comp[p] = c;
+        rdv[p] = r;
         flg[p] = fb;
     }
     // Longest-first launch order of the (rule, scenario block) items: a rule's
@@ -956,7 +982,6 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     bt->P = P;
     bt->T = T;
     bt->group_size = group_size;
-    bt->stk = max_depth <= 8 ? 8 : (max_depth <= 16 ? 16 : 32);
     bt->waves_per_item = inst->waves_per_item > 0 ? inst->waves_per_item
                                                    : default_waves((long long)P * ((T + 63) / 64));
     bt->rows_needed = P > 0 ? (int64_t)((P - 1) / group_size + 1) * T : 0;
@@ -975,7 +1000,7 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     if (up(o_ops, ops.data(), ops.size() * 4) != hipSuccess ||
         up(o_args, args.data(), args.size() * 4) != hipSuccess ||
         up(o_off, offs.data(), P * 4) != hipSuccess || up(o_len, lens.data(), P * 4) != hipSuccess ||
-        up(o_comp, comp.data(), P * 8) != hipSuccess || up(o_fl, flg.data(), P * 8) != hipSuccess ||
+        up(o_comp, rdv.data(), P * 8) != hipSuccess || up(o_fl, flg.data(), P * 8) != hipSuccess ||
         up(o_ord, order.data(), order.size() * 4) != hipSuccess) {
         (void)hipFree(base);
         delete bt;
@@ -985,10 +1010,12 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     bt->rs.args = (cip)(base + o_args);
     bt->rs.off = (cip)(base + o_off);
     bt->rs.len = (cip)(base + o_len);
-    bt->rs.comp = (cup)(base + o_comp);
+    bt->rs.rd = (cup)(base + o_comp);
     bt->rs.flags = (cup)(base + o_fl);
     bt->rs.order = order.empty() ? nullptr : (cip)(base + o_ord);
     bt->rs.P = P;
+    bt->rs.KL = max_k;
+    bt->rs.SL = std::max(1, max_depth - 1);
     *out = bt;
     return SIGP_OK;
 }
@@ -1026,7 +1053,18 @@ extern "C" sigp_status sigp_batch_shape(const sigp_batch *bt, int32_t *n_rules, 
     return SIGP_OK;
 }
 
-template <int NW, int STK>
+// LDS of one workgroup: the merge arrays plus W wavefronts' value stores and
+// stacks, kept within 64 KiB.  W = 1 always fits: (61 + 31) rows * 512 B.
+static constexpr size_t LDS_LIMIT = 65536, LDS_STATIC = 4 * 64 * (8 + 4);
+static size_t lds_per_wave(const sigp_batch *bt) {
+    return (size_t)(bt->rs.KL + bt->rs.SL) * 64 * sizeof(double);
+}
+static int fit_waves(const sigp_batch *bt, int w) {
+    while (w > 1 && LDS_STATIC + (size_t)w * lds_per_wave(bt) > LDS_LIMIT) w >>= 1;
+    return w;
+}
+
+template <int NW>
 static void launch_sim(const sigp_batch *bt, const double *sampled, double *out, hipStream_t stream) {
     const sigp_instance *I = bt->inst;
     ScenSpec S;
@@ -1035,7 +1073,8 @@ static void launch_sim(const sigp_batch *bt, const double *sampled, double *out,
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
     const unsigned items = (unsigned)((long long)bt->P * S.nblk);
-    sim_kernel<NW, STK><<<items, 64 * bt->waves_per_item, 0, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+    const int w = fit_waves(bt, bt->waves_per_item);
+    sim_kernel<NW><<<items, 64 * w, w * lds_per_wave(bt), stream>>>(I->dev, I->tabs, bt->rs, S, out);
 }
 
 extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,
@@ -1058,19 +1097,9 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
         pairs = bt->pairs_tmp;
     }
     const int nwb = nw_bucket(bt->inst->n);
-    if (bt->stk == 8) {
-        if (nwb == 4) launch_sim<4, 8>(bt, sampled_dev, pairs, s);
-        else if (nwb == 9) launch_sim<9, 8>(bt, sampled_dev, pairs, s);
-        else launch_sim<16, 8>(bt, sampled_dev, pairs, s);
-    } else if (bt->stk == 16) {
-        if (nwb == 4) launch_sim<4, 16>(bt, sampled_dev, pairs, s);
-        else if (nwb == 9) launch_sim<9, 16>(bt, sampled_dev, pairs, s);
-        else launch_sim<16, 16>(bt, sampled_dev, pairs, s);
-    } else {
-        if (nwb == 4) launch_sim<4, 32>(bt, sampled_dev, pairs, s);
-        else if (nwb == 9) launch_sim<9, 32>(bt, sampled_dev, pairs, s);
-        else launch_sim<16, 32>(bt, sampled_dev, pairs, s);
-    }
+    if (nwb == 4) launch_sim<4>(bt, sampled_dev, pairs, s);
+    else if (nwb == 9) launch_sim<9>(bt, sampled_dev, pairs, s);
+    else launch_sim<16>(bt, sampled_dev, pairs, s);
     HIP_TRY(hipGetLastError());
     if (out_mean_dev) {
         mean_kernel<<<(bt->P + 255) / 256, 256, 0, s>>>(pairs, bt->P, bt->T, out_mean_dev);
@@ -1096,6 +1125,7 @@ extern "C" sigp_status sigp_batch_autotune(sigp_batch *bt, const double *sampled
     float best_ms = 0.0f;
     sigp_status st = SIGP_OK;
     for (int k = 0; k < 3 && st == SIGP_OK; ++k) {
+        if (fit_waves(bt, cand[k]) != cand[k]) break;   // larger mappings do not fit LDS
         bt->waves_per_item = cand[k];
         (void)hipEventRecord(e0, s);
         st = sigp_batch_run(bt, sampled_dev, n_rows, bt->pairs_tmp, nullptr, stream);

@@ -48,9 +48,12 @@ struct DevInst {
 
 template <int NW>
 __device__ __forceinline__ uint32_t wsel(const uint32_t (&a)[NW], int w) {
-    uint32_t r = a[0];
+    // An and/or chain rather than a select chain: selects between loads of
+    // a[k] fold into one load from a computed address, which keeps the
+    // array in scratch memory instead of registers.
+    uint32_t r = 0;
 #pragma unroll
-    for (int k = 1; k < NW; ++k) r = (w == k) ? a[k] : r;
+    for (int k = 0; k < NW; ++k) r |= a[k] & (0u - (uint32_t)(w == k));
     return r;
 }
 
