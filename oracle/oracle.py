@@ -26,8 +26,9 @@ class OrcInstance(C.Structure):
 
 
 def build(reference: bool = True) -> None:
-    """Compile the oracle (and, where the reference sources exist, _ref)."""
-    targets = ["liboracle.so", "liboracle_count.so"] + (["ref"] if reference else [])
+    """Compile the oracle (and, where the reference sources exist, _ref: the
+    reference's CPU simulator and its GPU gate linked against libsigp.so)."""
+    targets = ["liboracle.so", "liboracle_count.so"] + (["ref", "gate"] if reference else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 
