@@ -81,7 +81,7 @@ __device__ __forceinline__ uint32_t bits8(const uint32_t (&a)[NW], int j0) {
 // NS (engine/sim_terms.hpp:18-34), MAXN (:76-93) and the first leg of
 // REACH2 (:101-111) from candidate c at time fin, one pass.
 template <int NW, bool NS, bool MX, bool R2>
-__device__ void scan_nsfam(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW],
+__device__ __forceinline__ void scan_nsfam(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW],
                            double &ns, double &mx, double &bv, int &bj) {
     const cdp tr = F.t0n + (size_t)(c + 1) * F.ldp, rr = F.r + (size_t)(c + 1) * F.ldp;
     double s = 0.0, m = 0.0, b = 0.0;
@@ -109,7 +109,7 @@ __device__ void scan_nsfam(const FusedTabs &F, int n, int c, double fin, const u
 // NSROB (engine/sim_terms.hpp:56-74), the first leg of REACH2ROB (:164-174)
 // and the first step of REACH-K-ROB (:455-468).
 template <int NW, bool NSR, bool R2R, bool RKR>
-__device__ void scan_robfam(const FusedTabs &F, int n, int c, double frob, double trob,
+__device__ __forceinline__ void scan_robfam(const FusedTabs &F, int n, int c, double frob, double trob,
                             const uint32_t (&vis)[NW], double &nsr, int &bjr, int &bkr) {
     const cdp tn = F.t2n + (size_t)(c + 1) * F.ldp, tp = F.t2 + (size_t)(c + 1) * F.ldp,
               rr = F.r + (size_t)(c + 1) * F.ldp;
@@ -145,7 +145,7 @@ __device__ void scan_robfam(const FusedTabs &F, int n, int c, double frob, doubl
 // REGRET (engine/sim_terms.hpp:36-54): neighbours feasible now (fnow) that
 // become infeasible after serving c.
 template <int NW>
-__device__ double scan_regret(const FusedTabs &F, int n, int c, double fin,
+__device__ __forceinline__ double scan_regret(const FusedTabs &F, int n, int c, double fin,
                               const uint32_t (&vis)[NW], const uint32_t (&fnow)[NW]) {
     const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp;
     double s = 0.0;
@@ -165,7 +165,7 @@ __device__ double scan_regret(const FusedTabs &F, int n, int c, double fin,
 // FRAGCNT, FRAGSCORE, DOWNSLACK and PBUST's best neighbour
 // (engine/sim_terms.hpp:544-562), after serving c at fin_c.
 template <int NW>
-__device__ void scan_struct(const FusedTabs &F, int n, int c, double fin_c, const uint32_t (&vis)[NW],
+__device__ __forceinline__ void scan_struct(const FusedTabs &F, int n, int c, double fin_c, const uint32_t (&vis)[NW],
                             const uint32_t (&fnow)[NW], double &fcnt, double &fsc, double &dsl,
                             int &bq) {
     const size_t row = (size_t)(c + 1) * F.ldp;
@@ -197,7 +197,7 @@ __device__ void scan_struct(const FusedTabs &F, int n, int c, double fin_c, cons
 
 // First step of REACH-K (engine/sim_terms.hpp:417-431) from c at t = fin_c.
 template <int NW>
-__device__ int scan_reach1(const FusedTabs &F, int n, int c, double t, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ int scan_reach1(const FusedTabs &F, int n, int c, double t, const uint32_t (&vis)[NW]) {
     const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp, rr = F.r + (size_t)(c + 1) * F.ldp;
     double bv = -1.0;
     int bi = -1;
@@ -219,7 +219,7 @@ __device__ int scan_reach1(const FusedTabs &F, int n, int c, double t, const uin
 // lane arrives before the window opens, so the common case is a plain
 // add of r (r * 1.0 == r); the division path runs one neighbour at a time.
 template <int NW>
-__device__ double scan_nstw(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double scan_nstw(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW]) {
     const size_t row = (size_t)(c + 1) * F.ldp;
     const cdp tr = F.t0n + row, rr = F.r + row, dd = F.d + row;
     double s = 0.0;
@@ -253,7 +253,7 @@ __device__ double scan_nstw(const FusedTabs &F, int n, int c, double fin, const 
 // comes from the per-instance duration-class table.  One neighbour at a
 // time: the body holds two erf evaluations.
 template <int NW>
-__device__ double scan_eloss(const DevInst &I, const FusedTabs &F, int c, double fin_c,
+__device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &F, int c, double fin_c,
                              const uint32_t (&vis)[NW], const uint32_t (&cnow)[NW]) {
     const double sdc = F.sdv[c];
     const double sdc2 = sdc * sdc;
@@ -294,7 +294,7 @@ __device__ double scan_eloss(const DevInst &I, const FusedTabs &F, int c, double
 // Second leg of REACH2 / REACH2ROB (engine/sim_terms.hpp:113-126 and
 // :176-189) from each lane's own first neighbour bj at time fj.
 template <int NW>
-__device__ double reach2_leg2(const DevInst &I, const FusedTabs &F, int i, int bj, double fj,
+__device__ __forceinline__ double reach2_leg2(const DevInst &I, const FusedTabs &F, int i, int bj, double fj,
                               const uint32_t (&vis)[NW], bool rob) {
     const int pj = bj + 1;
     double bv2 = 0.0, s2 = 0.0;
@@ -342,7 +342,7 @@ __device__ double reach2_leg2(const DevInst &I, const FusedTabs &F, int i, int b
 // REACH1/3/5 are prefixes of one path: returns the totals after
 // min(K, picks) for K = 1, 3, 5.
 template <int NW>
-__device__ void reach_path(const DevInst &I, const FusedTabs &F, int c, int best1, double t0,
+__device__ __forceinline__ void reach_path(const DevInst &I, const FusedTabs &F, int c, int best1, double t0,
                            const uint32_t (&vis)[NW], int K, bool rob, double *tot1,
                            double *tot3, double *tot5) {
     double total = 0.0;
@@ -414,7 +414,7 @@ __device__ void reach_path(const DevInst &I, const FusedTabs &F, int c, int best
 // st_c = max(ct + ta, open_c), fin_c = st_c + dur_c (structural, REACH-K,
 // ATRISK).
 template <int NW>
-__device__ void cand_scans(const DevInst &I, const FusedTabs &F, uint64_t fl, int kpath, int kpathr,
+__device__ __forceinline__ void cand_scans(const DevInst &I, const FusedTabs &F, uint64_t fl, int kpath, int kpathr,
                            int c, double fin, double st_c, double fin_c, const uint32_t (&vis)[NW],
                            const uint32_t (&fnow)[NW], const uint32_t (&cnow)[NW], CandVals &o) {
     const int n = I.n;

@@ -27,7 +27,7 @@ __device__ __constant__ static const uint64_t sigp_exp_tab[256] = SIGP_EXP_TAB_I
 __device__ __forceinline__ uint64_t sigp_bits(double x) { return (uint64_t)__double_as_longlong(x); }
 __device__ __forceinline__ double sigp_dbl(uint64_t u) { return __longlong_as_double((long long)u); }
 
-__device__ static double sigp_exp(double x) {
+__device__ __forceinline__ static double sigp_exp(double x) {
     const double inv_ln2_n = 0x1.71547652b82fep0 * 128.0;
     const double shift = 0x1.8p52;
     const double neg_ln2_hi_n = -0x1.62e42fefa0000p-8;
@@ -165,7 +165,7 @@ __device__ __forceinline__ double sigp_erf_expterm(double ax, double R, double S
     return sigp_exp(-z * z - 0.5625) * sigp_exp((z - ax) * (z + ax) + R / S);
 }
 
-__device__ static double sigp_erf(double x) {
+__device__ __forceinline__ static double sigp_erf(double x) {
     const int32_t hx = (int32_t)(sigp_bits(x) >> 32);
     const int32_t ix = hx & 0x7fffffff;
     if (ix >= 0x7ff00000) return (double)(1 - (int32_t)(((uint32_t)hx >> 31) << 1)) + 1.0 / x;
@@ -192,7 +192,7 @@ __device__ static double sigp_erf(double x) {
     return hx >= 0 ? 1.0 - r / ax : r / ax - 1.0;
 }
 
-__device__ static double sigp_erfc(double x) {
+__device__ __forceinline__ static double sigp_erfc(double x) {
     const int32_t hx = (int32_t)(sigp_bits(x) >> 32);
     const int32_t ix = hx & 0x7fffffff;
     if (ix >= 0x7ff00000) return (double)(((uint32_t)hx >> 31) << 1) + 1.0 / x;
@@ -263,7 +263,7 @@ __host__ __device__ __forceinline__ sigp_dd sigp_dd_recip_odd(int k) {   // 1/k 
 }
 
 // log(x) for finite x > 0 (normal or subnormal) as a double-double.
-__host__ __device__ static sigp_dd sigp_log_dd(double x) {
+__host__ __device__ __forceinline__ static sigp_dd sigp_log_dd(double x) {
     int e = 0;
     if (x < 0x1p-1022) { x *= 0x1p54; e = -54; }
     uint64_t u = __builtin_bit_cast(uint64_t, x);
@@ -289,7 +289,7 @@ __host__ __device__ static sigp_dd sigp_log_dd(double x) {
 }
 
 // exp of a double-double argument, |t| < 745.
-__host__ __device__ static double sigp_exp_dd(sigp_dd t) {
+__host__ __device__ __forceinline__ static double sigp_exp_dd(sigp_dd t) {
     const sigp_dd ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
     const double k = rint(t.hi * 1.4426950408889634);
     const double ln2_tail = 0x1.7b57a079a1934p-111;   // ln2 beyond the pair
@@ -317,7 +317,7 @@ __host__ __device__ static double sigp_exp_dd(sigp_dd t) {
 }
 
 // pow(x, y) for the rmod operator: x = max(a, 0) >= 0, y = 1 + clip(b, 0, 1).
-__host__ __device__ static double sigp_pow_dd(double x, double y) {
+__host__ __device__ __forceinline__ static double sigp_pow_dd(double x, double y) {
     if (y != y) return x == 1.0 ? 1.0 : y;
     if (x != x) return x;
     if (y == 0.0 || x == 1.0) return 1.0;

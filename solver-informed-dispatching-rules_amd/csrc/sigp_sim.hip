@@ -241,6 +241,9 @@ struct ScenSpec {
 // 64-lane row per compact slot index); the stack lives in LDS below them
 // (`st`, one row per level) with its top element in a register.  The stack
 // depth is wave-uniform, so every LDS access is one conflict-free row.
+// RMOD = false compiles out the rmod operator (opcode 6) for batches with no
+// such node; its double-double pow alone costs ~35 VGPRs of occupancy.
+template <bool RMOD>
 __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L,
                                             const double *tv, double *st) {
     double top = 0.0;
@@ -264,7 +267,7 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
             else if (o == 2) r = a * b;
             else if (o == 3) r = fabs(b) < 1e-10 ? 1.0 : a / b;
             else if (o == 4) r = a < b ? a : b;
-            else if (o == 5) r = a > b ? a : b;
+            else if (o == 5 || !RMOD) r = a > b ? a : b;
             else {
                 const double base = a > 0.0 ? a : 0.0;
                 double e = b;
@@ -285,7 +288,9 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
 // `rd` holds the slots the rule reads; slot s is stored at compact index
 // popcount(rd & (bit(s) - 1)) of the value store, as its leaves were
 // renumbered when the batch was built.
-template <int NW>
+// RARE = false compiles out the slots no scan family produces (44, 49, 50,
+// 53-57, 59, 60) for batches whose rules read none of them.
+template <int NW, bool RMOD, bool RARE>
 __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
                       uint64_t rd, uint64_t fl, int cur, double ct, int rem,
                       const uint32_t (&vis)[NW], int i0, int istep, double *tv, double *st,
@@ -407,7 +412,7 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
             PUT(42, (ta + di) / den);
         }
         if (rd & B(43)) PUT(43, v.r2rob - v.regret);
-        if (rd & B(44)) PUT(44, t_skiploss<NW>(I, i, cur, ct, vis));
+        if (RARE && (rd & B(44))) PUT(44, t_skiploss<NW>(I, i, cur, ct, vis));
         if (rd & B(45)) PUT(45, v.ns - v.nsrob);
         if (rd & B(46)) {
             double den = ta + I.d2d[i];
@@ -416,21 +421,21 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
         }
         if (rd & B(47)) PUT(47, v.rkr3);
         if (rd & B(48)) PUT(48, v.rkr5);
-        if (rd & B(49)) PUT(49, t_ns_alpha<NW>(I, i, fin, vis, false));
-        if (rd & B(50)) PUT(50, t_ns_alpha<NW>(I, i, fin, vis, true));
+        if (RARE && (rd & B(49))) PUT(49, t_ns_alpha<NW>(I, i, fin, vis, false));
+        if (RARE && (rd & B(50))) PUT(50, t_ns_alpha<NW>(I, i, fin, vis, true));
         if (rd & B(51)) PUT(51, fsc);
         if (rd & B(52)) PUT(52, nfr);
-        if (rd & B(53)) PUT(53, t_ns_rob_m<NW>(I, i, fin, di, vis, 0.5, true));
-        if (rd & B(54)) PUT(54, t_ns_rob_m<NW>(I, i, fin, di, vis, 2.0, true));
-        if (rd & B(55)) PUT(55, t_nnfwd<NW>(I, i, fin, vis));
-        if (rd & B(56)) PUT(56, t_nnk3<NW>(I, i, fin, vis));
-        if (rd & B(57)) PUT(57, t_nnchain<NW>(I, i, st_c, vis, 3));
+        if (RARE && (rd & B(53))) PUT(53, t_ns_rob_m<NW>(I, i, fin, di, vis, 0.5, true));
+        if (RARE && (rd & B(54))) PUT(54, t_ns_rob_m<NW>(I, i, fin, di, vis, 2.0, true));
+        if (RARE && (rd & B(55))) PUT(55, t_nnfwd<NW>(I, i, fin, vis));
+        if (RARE && (rd & B(56))) PUT(56, t_nnk3<NW>(I, i, fin, vis));
+        if (RARE && (rd & B(57))) PUT(57, t_nnchain<NW>(I, i, st_c, vis, 3));
         if (rd & B(58)) PUT(58, I.distT[cur] + I.d2d[i] - ta);   // dist[cur][0]
-        if (rd & B(59)) PUT(59, t_detour<NW>(I, i, cur, fin, ta, vis));
-        if (rd & B(60)) PUT(60, t_scorerank<NW>(I, i, fin, vis));
+        if (RARE && (rd & B(59))) PUT(59, t_detour<NW>(I, i, cur, fin, ta, vis));
+        if (RARE && (rd & B(60))) PUT(60, t_scorerank<NW>(I, i, fin, vis));
 
 #undef PUT
-        double pri = eval_rule(ops, args, L, tv, st);
+        double pri = eval_rule<RMOD>(ops, args, L, tv, st);
         if (pri != pri) pri = -1e300;
         if (pri > best_pri) { best_pri = pri; best = i; }
     }
@@ -446,8 +451,10 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
 // W wavefronts hold identical route state, so they leave the loop together.
 // Dynamic LDS: per wavefront KL + SL rows of 64 doubles (value store, then
 // stack), so a launch takes W * (KL + SL) * 512 bytes.
-template <int NW>
-__global__ __launch_bounds__(256) void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
+template <int NW, bool RMOD, bool RARE>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 256),
+                          amdgpu_waves_per_eu((NW == 4 && !RMOD && !RARE) ? 3 : 1)))
+void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
                                                   double *__restrict__ out) {
     __shared__ double lpri[4][64];
     __shared__ int lidx[4][64];
@@ -481,7 +488,7 @@ __global__ __launch_bounds__(256) void sim_kernel(DevInst I, FusedTabs F, RuleSe
         double bp = -1e300;
         if (running) {
             const int rem = I.m - veh - 1;
-            best = decide<NW>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis, wv, W, tv, st, bp);
+            best = decide<NW, RMOD, RARE>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis, wv, W, tv, st, bp);
         }
         if (W > 1) {
             lpri[wv][lane] = bp;
@@ -575,6 +582,8 @@ struct sigp_batch {
     int P = 0, T = 0, group_size = 1;
     int waves_per_item = 4;         // cooperating wavefronts per (rule, scenario block)
     int64_t rows_needed = 0;
+    bool has_rmod = false;          // some rule has an rmod node
+    bool has_rare = false;          // some rule reads a slot outside the scan families
     void *dbuf = nullptr;
     RuleSet rs{};
     double *pairs_tmp = nullptr;    // used when the caller wants means only
@@ -825,6 +834,11 @@ extern "C" sigp_status sigp_zstats(const sigp_instance_desc *desc, double *zp_ou
     return sigp_instance_destroy(I);
 }
 
+// Slots computed by their own per-candidate routines rather than the fused
+// scan families (decide's RARE variant).
+static constexpr uint64_t RARE_SLOTS = B(44) | B(49) | B(50) | B(53) | B(54) | B(55) | B(56) |
+                                       B(57) | B(59) | B(60);
+
 // Rule validation and the per-rule compute set.
 // A slot's value is live in the reference when (engine/sim_core.hpp:88-197):
 // slots 0-9 always; 11/14/30 when their own or their Z partner's mask bit is
@@ -893,6 +907,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     std::vector<int32_t> ops, args, offs(P), lens(P);
     std::vector<uint64_t> comp(P), rdv(P), flg(P);
     int max_depth = 1, max_k = 1;
+    bool any_rmod = false;
+    uint64_t any_rd = 0;
     for (int p = 0; p < P; ++p) {
         const int64_t o = rules->offsets[p], L = rules->lengths[p];
         if (o < 0 || L < 1 || o + L > rules->n_nodes)
@@ -914,6 +930,7 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
             } else if (op == -2) {
                 ++depth;
             } else if (op >= 0 && op <= 6) {
+                if (op == 6) any_rmod = true;
                 if (depth < 2)
                     return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack underflow");
                 --depth;
@@ -945,6 +962,7 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
         }
         comp[p] = c;
         rdv[p] = r;
+        any_rd |= r;
         flg[p] = fb;
     }
     // Longest-first launch order of the (rule, scenario block) items: a rule's
@@ -968,8 +986,7 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
             if (f & F_ATRISK) c += 8.0f;
             if (f & F_REACHK) c += 1.5f * (float)((f >> 40) & 7);
             if (f & F_REACHKR) c += 1.5f * (float)((f >> 43) & 7);
-            if (comp[p] & (B(44) | B(49) | B(50) | B(53) | B(54) | B(55) | B(56) | B(57) | B(59) | B(60)))
-                c += 4.0f;
+            if (comp[p] & RARE_SLOTS) c += 4.0f;
             cost[p] = c;
         }
         order.resize((size_t)P * nblk);
@@ -1015,6 +1032,8 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     bt->rs.order = order.empty() ? nullptr : (cip)(base + o_ord);
     bt->rs.P = P;
     bt->rs.KL = max_k;
+    bt->has_rmod = any_rmod;
+    bt->has_rare = (any_rd & RARE_SLOTS) != 0;
     bt->rs.SL = std::max(1, max_depth - 1);
     *out = bt;
     return SIGP_OK;
@@ -1074,7 +1093,14 @@ static void launch_sim(const sigp_batch *bt, const double *sampled, double *out,
     S.nblk = (bt->T + 63) / 64;
     const unsigned items = (unsigned)((long long)bt->P * S.nblk);
     const int w = fit_waves(bt, bt->waves_per_item);
-    sim_kernel<NW><<<items, 64 * w, w * lds_per_wave(bt), stream>>>(I->dev, I->tabs, bt->rs, S, out);
+    const unsigned lds = (unsigned)(w * lds_per_wave(bt));
+    if (bt->has_rmod) {
+        if (bt->has_rare) sim_kernel<NW, true, true><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+        else sim_kernel<NW, true, false><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+    } else {
+        if (bt->has_rare) sim_kernel<NW, false, true><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+        else sim_kernel<NW, false, false><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+    }
 }
 
 extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,

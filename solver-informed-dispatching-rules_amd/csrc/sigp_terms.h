@@ -76,7 +76,7 @@ __device__ __forceinline__ bool wtest(const uint32_t (&a)[NW], int idx) {
 
 // engine/sim_terms.hpp:18-34 (t_ns) with rk for score/t.
 template <int NW>
-__device__ double t_ns(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double t_ns(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     const cdp rrow = I.rk + (size_t)(i + 1) * I.n;
     double total = 0.0;
@@ -98,7 +98,7 @@ __device__ double t_ns(const DevInst &I, int i, double fin, const uint32_t (&vis
 // engine/sim_terms.hpp:56-74 (t_ns_rob, general = false) and :295-314
 // (t_ns_rob_m, general = true, ms = m * sigma).
 template <int NW>
-__device__ double t_ns_rob_m(const DevInst &I, int i, double fin, double dur_i,
+__device__ __forceinline__ double t_ns_rob_m(const DevInst &I, int i, double fin, double dur_i,
                              const uint32_t (&vis)[NW], double msig, bool general) {
     const double ms = general ? msig * kSigma : kSigma;
     const double frob = fin + dur_i * ms;
@@ -123,7 +123,7 @@ __device__ double t_ns_rob_m(const DevInst &I, int i, double fin, double dur_i,
 
 // engine/sim_terms.hpp:76-93
 template <int NW>
-__device__ double t_maxn(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double t_maxn(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     const cdp rrow = I.rk + (size_t)(i + 1) * I.n;
     double best = 0.0;
@@ -147,7 +147,7 @@ __device__ double t_maxn(const DevInst &I, int i, double fin, const uint32_t (&v
 // second leg starts from the lane's best first neighbour: its row is a
 // per-lane gather through the transposed tables.
 template <int NW>
-__device__ double t_reach2x(const DevInst &I, int i, double fin, double dur_i,
+__device__ __forceinline__ double t_reach2x(const DevInst &I, int i, double fin, double dur_i,
                             const uint32_t (&vis)[NW], bool rob) {
     const double k1 = 1.0 + kSigma;
     const double f0 = rob ? fin + dur_i * kSigma : fin;
@@ -196,7 +196,7 @@ __device__ double t_reach2x(const DevInst &I, int i, double fin, double dur_i,
 
 // engine/sim_terms.hpp:36-54: the "now" half of each test is fnow[j].
 template <int NW>
-__device__ double t_regret(const DevInst &I, int i, double fin_i, const uint32_t (&vis)[NW],
+__device__ __forceinline__ double t_regret(const DevInst &I, int i, double fin_i, const uint32_t (&vis)[NW],
                            const uint32_t (&fnow)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     double total = 0.0;
@@ -217,7 +217,7 @@ __device__ double t_regret(const DevInst &I, int i, double fin_i, const uint32_t
 // engine/sim_terms.hpp:129-154: the first scan runs from the lane's current
 // POI (column gathers through distT / rkT).
 template <int NW>
-__device__ double t_skiploss(const DevInst &I, int i, int cur, double ct,
+__device__ __forceinline__ double t_skiploss(const DevInst &I, int i, int cur, double ct,
                              const uint32_t (&vis)[NW]) {
     double bv = 0.0, fq = 0.0;
     int bq = -1;
@@ -245,7 +245,7 @@ __device__ double t_skiploss(const DevInst &I, int i, int cur, double ct,
 
 // engine/sim_terms.hpp:192-213
 template <int NW>
-__device__ double t_nstw(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double t_nstw(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     const cdp rrow = I.rk + (size_t)(i + 1) * I.n;
     double total = 0.0;
@@ -285,7 +285,7 @@ __device__ __forceinline__ double norm_cdf(double z) {
 // is cnow[j]; pow(var, 0.5) comes from the per-instance duration-class table
 // (the identical libm call, made once per instance).
 template <int NW>
-__device__ double t_eloss(const DevInst &I, int c, double fin_c, const uint32_t (&vis)[NW],
+__device__ __forceinline__ double t_eloss(const DevInst &I, int c, double fin_c, const uint32_t (&vis)[NW],
                           const uint32_t (&cnow)[NW]) {
     const double sdc = I.sdv[c];
     const double sdc2 = sdc * sdc;
@@ -323,7 +323,7 @@ __device__ double t_eloss(const DevInst &I, int c, double fin_c, const uint32_t 
 // engine/sim_terms.hpp:259-275 (half = false: score/t^2) and :277-293
 // (half = true: score/sqrt(t)).
 template <int NW>
-__device__ double t_ns_alpha(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW],
+__device__ __forceinline__ double t_ns_alpha(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW],
                              bool half) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     double total = 0.0;
@@ -344,7 +344,7 @@ __device__ double t_ns_alpha(const DevInst &I, int i, double fin, const uint32_t
 
 // engine/sim_terms.hpp:316-334
 template <int NW>
-__device__ double t_nnfwd(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double t_nnfwd(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     double dmin = -1.0;
     uint32_t vw = 0;
@@ -365,7 +365,7 @@ __device__ double t_nnfwd(const DevInst &I, int i, double fin, const uint32_t (&
 
 // engine/sim_terms.hpp:336-359
 template <int NW>
-__device__ double t_nnk3(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double t_nnk3(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     double d1 = -1.0, d2 = -1.0, d3 = -1.0;
     uint32_t vw = 0;
@@ -391,7 +391,7 @@ __device__ double t_nnk3(const DevInst &I, int i, double fin, const uint32_t (&v
 
 // engine/sim_terms.hpp:361-381 (dist(current, j) is a column gather).
 template <int NW>
-__device__ double t_detour(const DevInst &I, int i, int cur, double fin, double ta_i,
+__device__ __forceinline__ double t_detour(const DevInst &I, int i, int cur, double fin, double ta_i,
                            const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     double total = 0.0, cnt = 0.0;
@@ -414,7 +414,7 @@ __device__ double t_detour(const DevInst &I, int i, int cur, double fin, double 
 
 // engine/sim_terms.hpp:383-401
 template <int NW>
-__device__ double t_scorerank(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
+__device__ __forceinline__ double t_scorerank(const DevInst &I, int i, double fin, const uint32_t (&vis)[NW]) {
     const cdp drow = I.dist + (size_t)(i + 1) * I.ld + 1;
     const double si = I.score[i];
     double below = 0.0, tot = 0.0;
@@ -440,7 +440,7 @@ __device__ double t_scorerank(const DevInst &I, int i, double fin, const uint32_
 // The first step scans row c+1 (uniform); later steps scan the lane's own
 // waypoint row through the transposed tables.
 template <int NW>
-__device__ double t_reach_k(const DevInst &I, int c, double st_c, const uint32_t (&vis)[NW],
+__device__ __forceinline__ double t_reach_k(const DevInst &I, int c, double st_c, const uint32_t (&vis)[NW],
                             int K, bool rob) {
     const double k1 = 1.0 + kSigma;
     uint32_t scr[NW];
@@ -497,7 +497,7 @@ __device__ double t_reach_k(const DevInst &I, int c, double st_c, const uint32_t
 
 // engine/sim_terms.hpp:481-519
 template <int NW>
-__device__ double t_nnchain(const DevInst &I, int c, double st_c, const uint32_t (&vis)[NW], int K) {
+__device__ __forceinline__ double t_nnchain(const DevInst &I, int c, double st_c, const uint32_t (&vis)[NW], int K) {
     uint32_t scr[NW];
 #pragma unroll
     for (int w = 0; w < NW; ++w) scr[w] = vis[w];
@@ -537,7 +537,7 @@ __device__ double t_nnchain(const DevInst &I, int c, double st_c, const uint32_t
 // engine/sim_terms.hpp:533-580: FRAGCNT, FRAGSCORE, DOWNSLACK, PBUST.
 // feas_now(q) is fnow[q]; feas_after(q) is tested from POI c+1 at fin_c.
 template <int NW>
-__device__ void struct_feats(const DevInst &I, int c, double fin_c, const uint32_t (&vis)[NW],
+__device__ __forceinline__ void struct_feats(const DevInst &I, int c, double fin_c, const uint32_t (&vis)[NW],
                              const uint32_t (&fnow)[NW], double out[4]) {
     const cdp drow = I.dist + (size_t)(c + 1) * I.ld + 1;
     const cdp rrow = I.rk + (size_t)(c + 1) * I.n;
