@@ -49,13 +49,6 @@ struct FusedTabs {
     cdp open, close, dur, d2d, score, durrob, sdv;   // customer arrays, padded to ldp
 };
 
-// Result of the candidate scans (per lane).
-struct CandVals {
-    double ns, regret, nsrob, maxn, reach2, nstw;
-    double fcnt, fscore, dslack, pbust;
-    double rk1, rk3, rk5, rkr3, rkr5, eloss, r2rob;
-};
-
 __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
 
 struct D8 { double v[8]; };
@@ -410,86 +403,125 @@ __device__ __forceinline__ void reach_path(const DevInst &I, const FusedTabs &F,
     }
 }
 
-// Every scan the rule needs for candidate c.  fin = ct + tfv (NS family),
+// Store terminal slot s of the rule (read set rd) in the wavefront's LDS
+// value store: row popcount(rd & (bit(s) - 1)), one double per lane.
+__device__ __forceinline__ void put_slot(double *tv, uint64_t rd, int s, double v) {
+    if ((rd >> s) & 1ull) tv[__builtin_popcountll(rd & ((1ull << s) - 1ull)) * 64] = v;
+}
+
+// Every scan the rule needs for candidate c, each family's slots stored as
+// soon as it finishes (engine/sim_core.hpp:99-198 for the slot formulas), so
+// that only the four values the derived slots combine (NS, NSROB, REACH2ROB,
+// REGRET) stay live across families.  fin = ct + tfv (NS family),
 // st_c = max(ct + ta, open_c), fin_c = st_c + dur_c (structural, REACH-K,
-// ATRISK).
+// ATRISK); ta = travel time from the current POI.
 template <int NW>
-__device__ __forceinline__ void cand_scans(const DevInst &I, const FusedTabs &F, uint64_t fl, int kpath, int kpathr,
-                           int c, double fin, double st_c, double fin_c, const uint32_t (&vis)[NW],
-                           const uint32_t (&fnow)[NW], const uint32_t (&cnow)[NW], CandVals &o) {
+__device__ __forceinline__ void cand_terms(const DevInst &I, const FusedTabs &F, uint64_t fl,
+                                           uint64_t rd, int kpath, int kpathr, int c, double ta,
+                                           double fin, double st_c, double fin_c,
+                                           const uint32_t (&vis)[NW], const uint32_t (&fnow)[NW],
+                                           const uint32_t (&cnow)[NW], double *tv) {
     const int n = I.n;
     const double dur_c = F.dur[c];
-    o.ns = o.maxn = o.reach2 = 0.0;
-    double bv2 = 0.0;
-    int bj2 = -1;
-    switch ((int)(fl & (F_NS | F_MAXN | F_REACH2))) {
-    case (int)F_NS: scan_nsfam<NW, true, false, false>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    case (int)F_MAXN: scan_nsfam<NW, false, true, false>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    case (int)F_REACH2: scan_nsfam<NW, false, false, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    case (int)(F_NS | F_MAXN): scan_nsfam<NW, true, true, false>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    case (int)(F_NS | F_REACH2): scan_nsfam<NW, true, false, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    case (int)(F_MAXN | F_REACH2): scan_nsfam<NW, false, true, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    case (int)(F_NS | F_MAXN | F_REACH2): scan_nsfam<NW, true, true, true>(F, n, c, fin, vis, o.ns, o.maxn, bv2, bj2); break;
-    default: break;
-    }
-    if ((fl & F_REACH2) && bj2 >= 0) {   // engine/sim_terms.hpp:110,112-126
-        const double arr = fin + F.d[(size_t)(c + 1) * F.ldp + bj2];
-        const double st = arr > F.open[bj2] ? arr : F.open[bj2];
-        o.reach2 = reach2_leg2<NW>(I, F, c, bj2, st + F.dur[bj2], vis, false);
-    }
-
-    const double frob = fin + dur_c * kSigma;   // engine/sim_terms.hpp:61,161
-    const double trob = st_c + F.durrob[c];     // engine/sim_terms.hpp:453
-    o.nsrob = o.r2rob = 0.0;
-    int bjr = -1, bkr = -1;
-    switch ((int)(fl & (F_NSROB | F_R2ROB | F_REACHKR))) {
-    case (int)F_NSROB: scan_robfam<NW, true, false, false>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    case (int)F_R2ROB: scan_robfam<NW, false, true, false>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    case (int)F_REACHKR: scan_robfam<NW, false, false, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    case (int)(F_NSROB | F_R2ROB): scan_robfam<NW, true, true, false>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    case (int)(F_NSROB | F_REACHKR): scan_robfam<NW, true, false, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    case (int)(F_R2ROB | F_REACHKR): scan_robfam<NW, false, true, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    case (int)(F_NSROB | F_R2ROB | F_REACHKR): scan_robfam<NW, true, true, true>(F, n, c, frob, trob, vis, o.nsrob, bjr, bkr); break;
-    default: break;
-    }
-    if ((fl & F_R2ROB) && bjr >= 0) {   // engine/sim_terms.hpp:173,175-189
-        const double arr = frob + F.d[(size_t)(c + 1) * F.ldp + bjr];
-        const double st = arr > F.open[bjr] ? arr : F.open[bjr];
-        o.r2rob = reach2_leg2<NW>(I, F, c, bjr, st + F.durrob[bjr], vis, true);
-    }
-    o.rkr3 = o.rkr5 = 0.0;
-    if (fl & F_REACHKR) {
-        double t1;
-        reach_path<NW>(I, F, c, bkr, trob, vis, kpathr, true, &t1, &o.rkr3, &o.rkr5);
+    double ns = 0.0, nsrob = 0.0, r2rob = 0.0, regret = 0.0;
+    if (fl & (F_NS | F_MAXN | F_REACH2)) {
+        double maxn = 0.0, bv2 = 0.0, reach2 = 0.0;
+        int bj2 = -1;
+        switch ((int)(fl & (F_NS | F_MAXN | F_REACH2))) {
+        case (int)F_NS: scan_nsfam<NW, true, false, false>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        case (int)F_MAXN: scan_nsfam<NW, false, true, false>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        case (int)F_REACH2: scan_nsfam<NW, false, false, true>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        case (int)(F_NS | F_MAXN): scan_nsfam<NW, true, true, false>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        case (int)(F_NS | F_REACH2): scan_nsfam<NW, true, false, true>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        case (int)(F_MAXN | F_REACH2): scan_nsfam<NW, false, true, true>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        default: scan_nsfam<NW, true, true, true>(F, n, c, fin, vis, ns, maxn, bv2, bj2); break;
+        }
+        put_slot(tv, rd, 10, ns);
+        put_slot(tv, rd, 13, maxn);
+        if ((fl & F_REACH2) && bj2 >= 0) {   // engine/sim_terms.hpp:110,112-126
+            const double arr = fin + F.d[(size_t)(c + 1) * F.ldp + bj2];
+            const double st = arr > F.open[bj2] ? arr : F.open[bj2];
+            reach2 = reach2_leg2<NW>(I, F, c, bj2, st + F.dur[bj2], vis, false);
+        }
+        put_slot(tv, rd, 14, reach2);
+        put_slot(tv, rd, 32, (reach2 - I.zp[1]) / I.zp[6]);
     }
 
-    o.regret = (fl & F_REGRET) ? scan_regret<NW>(F, n, c, fin, vis, fnow) : 0.0;
+    if (fl & (F_NSROB | F_R2ROB | F_REACHKR)) {
+        const double frob = fin + dur_c * kSigma;   // engine/sim_terms.hpp:61,161
+        const double trob = st_c + F.durrob[c];     // engine/sim_terms.hpp:453
+        int bjr = -1, bkr = -1;
+        switch ((int)(fl & (F_NSROB | F_R2ROB | F_REACHKR))) {
+        case (int)F_NSROB: scan_robfam<NW, true, false, false>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        case (int)F_R2ROB: scan_robfam<NW, false, true, false>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        case (int)F_REACHKR: scan_robfam<NW, false, false, true>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        case (int)(F_NSROB | F_R2ROB): scan_robfam<NW, true, true, false>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        case (int)(F_NSROB | F_REACHKR): scan_robfam<NW, true, false, true>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        case (int)(F_R2ROB | F_REACHKR): scan_robfam<NW, false, true, true>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        default: scan_robfam<NW, true, true, true>(F, n, c, frob, trob, vis, nsrob, bjr, bkr); break;
+        }
+        put_slot(tv, rd, 12, nsrob);
+        if ((fl & F_R2ROB) && bjr >= 0) {   // engine/sim_terms.hpp:173,175-189
+            const double arr = frob + F.d[(size_t)(c + 1) * F.ldp + bjr];
+            const double st = arr > F.open[bjr] ? arr : F.open[bjr];
+            r2rob = reach2_leg2<NW>(I, F, c, bjr, st + F.durrob[bjr], vis, true);
+        }
+        put_slot(tv, rd, 41, r2rob);
+        if (fl & F_REACHKR) {
+            double t1, rkr3, rkr5;
+            reach_path<NW>(I, F, c, bkr, trob, vis, kpathr, true, &t1, &rkr3, &rkr5);
+            put_slot(tv, rd, 47, rkr3);
+            put_slot(tv, rd, 48, rkr5);
+        }
+    }
+    put_slot(tv, rd, 45, ns - nsrob);
+    if (rd & (1ull << 46)) {
+        double den = ta + F.d2d[c];
+        if (den < 1.0) den = 1.0;
+        put_slot(tv, rd, 46, (F.score[c] + nsrob) / den);
+    }
 
-    o.fcnt = o.fscore = o.dslack = o.pbust = 0.0;
+    if (fl & F_REGRET) regret = scan_regret<NW>(F, n, c, fin, vis, fnow);
+    put_slot(tv, rd, 11, regret);
+    put_slot(tv, rd, 33, (regret - I.zp[2]) / I.zp[7]);
+    put_slot(tv, rd, 43, r2rob - regret);
+
     if (fl & F_STRUCT) {
+        double fcnt, fscore, dsl, pbust = 0.0;
         int bq = -1;
-        double dsl;
-        scan_struct<NW>(F, n, c, fin_c, vis, fnow, o.fcnt, o.fscore, dsl, bq);
-        o.dslack = dsl > 1e17 ? 0.0 : dsl;
+        scan_struct<NW>(F, n, c, fin_c, vis, fnow, fcnt, fscore, dsl, bq);
         if (bq >= 0) {   // engine/sim_terms.hpp:565-575
             const double ab = fin_c + F.d[(size_t)(c + 1) * F.ldp + bq];
             const double sd = kSigma * dur_c;
             if (sd > 1e-12) {
                 const double z = (F.close[bq] - ab) / (sd * kSqrt2);
-                o.pbust = 0.5 * sigp_erfc(z);
+                pbust = 0.5 * sigp_erfc(z);
             } else {
-                o.pbust = ab > F.close[bq] ? 1.0 : 0.0;
+                pbust = ab > F.close[bq] ? 1.0 : 0.0;
             }
         }
+        put_slot(tv, rd, 17, fcnt);
+        put_slot(tv, rd, 18, fscore);
+        put_slot(tv, rd, 19, dsl > 1e17 ? 0.0 : dsl);
+        put_slot(tv, rd, 20, pbust);
+        put_slot(tv, rd, 28, fscore / (ns + 1.0));
+        put_slot(tv, rd, 29, fscore - ns);
     }
 
-    o.rk1 = o.rk3 = o.rk5 = 0.0;
     if (fl & F_REACHK) {
+        double rk1, rk3, rk5;
         const int b1 = scan_reach1<NW>(F, n, c, fin_c, vis);
-        reach_path<NW>(I, F, c, b1, fin_c, vis, kpath, false, &o.rk1, &o.rk3, &o.rk5);
+        reach_path<NW>(I, F, c, b1, fin_c, vis, kpath, false, &rk1, &rk3, &rk5);
+        put_slot(tv, rd, 21, rk1);
+        put_slot(tv, rd, 22, rk3);
+        put_slot(tv, rd, 23, rk5);
     }
-    o.nstw = (fl & F_NSTW) ? scan_nstw<NW>(F, n, c, fin, vis) : 0.0;
-    o.eloss = (fl & F_ATRISK) ? scan_eloss<NW>(I, F, c, fin_c, vis, cnow) : 0.0;
+    if (fl & F_NSTW) put_slot(tv, rd, 15, scan_nstw<NW>(F, n, c, fin, vis));
+    if (fl & F_ATRISK) {
+        const double eloss = scan_eloss<NW>(I, F, c, fin_c, vis, cnow);
+        put_slot(tv, rd, 30, eloss);
+        put_slot(tv, rd, 34, (eloss - I.zp[3]) / I.zp[8]);
+    }
 }
 
 }  // namespace sigp

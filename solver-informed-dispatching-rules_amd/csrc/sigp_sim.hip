@@ -338,9 +338,20 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
             if ((m >> (j & 31)) & 1u) fsc += I.score[j];
         }
     }
-    const double vfrac = nvis / I.n;
-    const double sat = I.tmax > 0.0 ? ct / I.tmax : 0.0;
     const int kpath = (int)((fl >> 40) & 7), kpathr = (int)((fl >> 43) & 7);
+    // Slots that do not depend on the candidate: stored once per decision.
+    put_slot(tv, rd, 9, rem * I.tmax + (I.tmax - ct));
+    put_slot(tv, rd, 24, I.tmax > 0.0 ? ct / I.tmax : 0.0);
+    put_slot(tv, rd, 25, nvis / I.n);
+    put_slot(tv, rd, 26, nfeas);
+    put_slot(tv, rd, 27, (double)rem);
+    put_slot(tv, rd, 36, 0.1);
+    put_slot(tv, rd, 37, 0.2);
+    put_slot(tv, rd, 38, 0.3);
+    put_slot(tv, rd, 39, 0.4);
+    put_slot(tv, rd, 40, 0.5);
+    put_slot(tv, rd, 51, fsc);
+    put_slot(tv, rd, 52, nfr);
 
     int best = -1;
     double best_pri = -1e300;
@@ -356,81 +367,39 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
         const double tc = I.close[i] - ct;
         const double tsv = to > ta ? to : ta;
         const double tfv = tsv + di;
-#define PUT(sl, val) tv[__builtin_popcountll(rd & (B(sl) - 1)) * 64] = (val)
-        if (rd & B(0)) PUT(0, I.score[i]);
-        if (rd & B(1)) PUT(1, di);
-        if (rd & B(2)) PUT(2, to);
-        if (rd & B(3)) PUT(3, tc);
-        if (rd & B(4)) PUT(4, ta);
-        if (rd & B(5)) PUT(5, I.d2d[i]);
-        if (rd & B(6)) PUT(6, tsv);
-        if (rd & B(7)) PUT(7, tfv);
-        if (rd & B(8)) PUT(8, tc - ta);
-        if (rd & B(9)) PUT(9, rem * I.tmax + (I.tmax - ct));
-        const double fin = ct + tfv;
-        const double arr = ct + ta;
-        const double st_c = arr > oi ? arr : oi;
-        const double fin_c = st_c + di;
-
-        CandVals v{};
-        if (fl & 0x7FFull) cand_scans<NW>(I, F, fl, kpath, kpathr, i, fin, st_c, fin_c, vis, fnow, cnow, v);
-        if (rd & B(10)) PUT(10, v.ns);
-        if (rd & B(11)) PUT(11, v.regret);
-        if (rd & B(12)) PUT(12, v.nsrob);
-        if (rd & B(13)) PUT(13, v.maxn);
-        if (rd & B(14)) PUT(14, v.reach2);
-        if (rd & B(15)) PUT(15, v.nstw);
-        if (rd & B(16)) PUT(16, t_rand(i));
-        if (rd & B(17)) PUT(17, v.fcnt);
-        if (rd & B(18)) PUT(18, v.fscore);
-        if (rd & B(19)) PUT(19, v.dslack);
-        if (rd & B(20)) PUT(20, v.pbust);
-        if (rd & B(21)) PUT(21, v.rk1);
-        if (rd & B(22)) PUT(22, v.rk3);
-        if (rd & B(23)) PUT(23, v.rk5);
-        if (rd & B(24)) PUT(24, sat);
-        if (rd & B(25)) PUT(25, vfrac);
-        if (rd & B(26)) PUT(26, nfeas);
-        if (rd & B(27)) PUT(27, (double)rem);
-        if (rd & B(28)) PUT(28, v.fscore / (v.ns + 1.0));
-        if (rd & B(29)) PUT(29, v.fscore - v.ns);
-        if (rd & B(30)) PUT(30, v.eloss);
-        if (rd & B(31)) PUT(31, (tc - I.zp[0]) / I.zp[5]);
-        if (rd & B(32)) PUT(32, (v.reach2 - I.zp[1]) / I.zp[6]);
-        if (rd & B(33)) PUT(33, (v.regret - I.zp[2]) / I.zp[7]);
-        if (rd & B(34)) PUT(34, (v.eloss - I.zp[3]) / I.zp[8]);
-        if (rd & B(35)) PUT(35, (I.d2d[i] - I.zp[4]) / I.zp[9]);
-        if (rd & B(36)) PUT(36, 0.1);
-        if (rd & B(37)) PUT(37, 0.2);
-        if (rd & B(38)) PUT(38, 0.3);
-        if (rd & B(39)) PUT(39, 0.4);
-        if (rd & B(40)) PUT(40, 0.5);
-        if (rd & B(41)) PUT(41, v.r2rob);
+#define PUT(sl, val) put_slot(tv, rd, sl, val)
+        PUT(0, I.score[i]);
+        PUT(1, di);
+        PUT(2, to);
+        PUT(3, tc);
+        PUT(4, ta);
+        PUT(5, I.d2d[i]);
+        PUT(6, tsv);
+        PUT(7, tfv);
+        PUT(8, tc - ta);
+        PUT(16, t_rand(i));
+        PUT(31, (tc - I.zp[0]) / I.zp[5]);
+        PUT(35, (I.d2d[i] - I.zp[4]) / I.zp[9]);
         if (rd & B(42)) {
             double den = I.tmax - ct;
             if (den < 1.0) den = 1.0;
             PUT(42, (ta + di) / den);
         }
-        if (rd & B(43)) PUT(43, v.r2rob - v.regret);
+        PUT(58, I.distT[cur] + I.d2d[i] - ta);   // dist[cur][0]
+        const double fin = ct + tfv;
+        const double arr = ct + ta;
+        const double st_c = arr > oi ? arr : oi;
+        const double fin_c = st_c + di;
+
+        if (fl & 0x7FFull) cand_terms<NW>(I, F, fl, rd, kpath, kpathr, i, ta, fin, st_c, fin_c, vis, fnow, cnow, tv);
         if (RARE && (rd & B(44))) PUT(44, t_skiploss<NW>(I, i, cur, ct, vis));
-        if (rd & B(45)) PUT(45, v.ns - v.nsrob);
-        if (rd & B(46)) {
-            double den = ta + I.d2d[i];
-            if (den < 1.0) den = 1.0;
-            PUT(46, (I.score[i] + v.nsrob) / den);
-        }
-        if (rd & B(47)) PUT(47, v.rkr3);
-        if (rd & B(48)) PUT(48, v.rkr5);
         if (RARE && (rd & B(49))) PUT(49, t_ns_alpha<NW>(I, i, fin, vis, false));
         if (RARE && (rd & B(50))) PUT(50, t_ns_alpha<NW>(I, i, fin, vis, true));
-        if (rd & B(51)) PUT(51, fsc);
-        if (rd & B(52)) PUT(52, nfr);
         if (RARE && (rd & B(53))) PUT(53, t_ns_rob_m<NW>(I, i, fin, di, vis, 0.5, true));
         if (RARE && (rd & B(54))) PUT(54, t_ns_rob_m<NW>(I, i, fin, di, vis, 2.0, true));
         if (RARE && (rd & B(55))) PUT(55, t_nnfwd<NW>(I, i, fin, vis));
         if (RARE && (rd & B(56))) PUT(56, t_nnk3<NW>(I, i, fin, vis));
         if (RARE && (rd & B(57))) PUT(57, t_nnchain<NW>(I, i, st_c, vis, 3));
-        if (rd & B(58)) PUT(58, I.distT[cur] + I.d2d[i] - ta);   // dist[cur][0]
         if (RARE && (rd & B(59))) PUT(59, t_detour<NW>(I, i, cur, fin, ta, vis));
         if (RARE && (rd & B(60))) PUT(60, t_scorerank<NW>(I, i, fin, vis));
 
@@ -453,7 +422,7 @@ __device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const
 // stack), so a launch takes W * (KL + SL) * 512 bytes.
 template <int NW, bool RMOD, bool RARE>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 256),
-                          amdgpu_waves_per_eu((NW == 4 && !RMOD && !RARE) ? 3 : 1)))
+                          amdgpu_waves_per_eu(1)))
 void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
                                                   double *__restrict__ out) {
     __shared__ double lpri[4][64];
