@@ -90,6 +90,19 @@ def cpu_baseline(wl, zp, target_s=15.0):
                       % (nr, max(1, wl.packed.n_rules // nr), ns, dt)}
 
 
+def describe(kind, wl):
+    P, T = wl.packed.n_rules, wl.n_scen
+    if kind == "plain":
+        return ("plain method, small: 1 uniform instance n=100 m=2, %d rules (ramped half-and-half, "
+                "depth 2-6 levels, EXTRA-18 terminals) x %d Gamma(4, mean 10) scenarios" % (P, T))
+    if kind == "gp-generation":
+        return ("one GP generation: clustered instance n=100 m=3, %d rules (ramped half-and-half, "
+                "depth 3-9 levels, NS-11 + EXTRA-18) x %d Gamma(4) scenarios" % (P, T))
+    return ("one launch of the reference GP loop: clustered instance n=100 m=3, %d seeds x %d rules "
+            "(NS-11 + EXTRA-18, depth 3-9 levels) x 1 scenario each" % (P // wl.group_size,
+                                                                      wl.group_size))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,10 +110,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rules", type=int, default=1024)
     ap.add_argument("--scenarios", type=int, default=500)
+    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference"], default="plain",
+                    help="plain: the headline workload; gp-generation: one generation of the "
+                         "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
+                         "gp-reference: one launch of the reference's GP loop (30 seeds x 1,024 "
+                         "rules x 1 scenario)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--waves", type=int, default=0,
-                    help="wavefronts per (rule, scenario block): 1, 2 or 4; 0 = autotune")
+                    help="wavefronts per workgroup: 1, 2 or 4; 0 = autotune")
+    ap.add_argument("--mapping", choices=["scenarios", "candidates"], default="scenarios",
+                    help="lane mapping used with --waves")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
@@ -111,31 +131,39 @@ def main():
 
     import torch
     from sigp.api import FactoryContext, device_info
-    from sigp.workloads import plain_small
+    from sigp.workloads import gp_generation, gp_reference_generation, plain_small
 
     t0 = time.perf_counter()
-    wl = plain_small(n_rules=args.rules, n_scen=args.scenarios)
+    if args.workload == "plain":
+        wl = plain_small(n_rules=args.rules, n_scen=args.scenarios)
+    elif args.workload == "gp-generation":
+        wl = gp_generation()
+    else:
+        wl = gp_reference_generation()
     log("workload built in %.1fs: %s" % (time.perf_counter() - t0, wl.name))
     dev = torch.device("cuda:0")
     name, n_cu, clk = device_info()
     log("device", name, n_cu, "CUs")
     ctx = FactoryContext(wl.inst)
     P, T = wl.packed.n_rules, wl.n_scen
-    batch = ctx.batch(wl.packed, T, wl.mask)
+    batch = ctx.batch(wl.packed, T, wl.mask, group_size=wl.group_size or None)
+    n_rows = wl.sampled.shape[0]
     sampled = torch.from_numpy(wl.sampled).to(dev)
     pairs = torch.zeros(P * T, dtype=torch.float64, device=dev)
     means = torch.zeros(P, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
 
     def step():
-        batch.run(sampled.data_ptr(), T, pairs.data_ptr(), means.data_ptr(), stream.cuda_stream)
+        batch.run(sampled.data_ptr(), n_rows, pairs.data_ptr(), means.data_ptr(), stream.cuda_stream)
 
     if args.waves:
-        batch.configure(args.waves)
+        mapping = 1 if args.mapping == "scenarios" else 2
+        batch.configure(args.waves, mapping)
         waves, how = args.waves, "fixed"
     else:
-        waves, how = batch.autotune(sampled.data_ptr(), T, stream.cuda_stream), "autotuned"
-    log("%s mapping: %d wavefronts per (rule, 64-scenario block)" % (how, waves))
+        (waves, mapping), how = batch.autotune(sampled.data_ptr(), n_rows, stream.cuda_stream), "autotuned"
+    lanes = "64-scenario block" if mapping == 1 else "(rule, scenario) simulation"
+    log("%s mapping: %d wavefronts per %s" % (how, waves, lanes))
 
     for _ in range(args.warmup):
         step()
@@ -167,7 +195,7 @@ def main():
                for _ in range(args.steps)]
         for a, b in kev:
             a.record(stream)
-            batch.run(sampled.data_ptr(), T, pairs.data_ptr(), None, stream.cuda_stream)
+            batch.run(sampled.data_ptr(), n_rows, pairs.data_ptr(), None, stream.cuda_stream)
             b.record(stream)
         torch.cuda.synchronize()
         k_ms = float(np.mean([a.elapsed_time(b) for a, b in kev]))
@@ -198,12 +226,11 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain method, small: 1 uniform instance n=100 m=2, %d rules "
-                               "(ramped half-and-half, depth 2-6 levels, EXTRA-18 terminals) x %d "
-                               "Gamma(4, mean 10) scenarios" % (P, T),
+        "config": {"workload": describe(args.workload, wl),
                    "rules": P, "scenarios": T, "n": wl.inst.n, "m": wl.inst.m,
-                   "parallelism": "1 GPU; workgroup = rule x 64 scenarios, %d cooperating "
-                                  "wavefronts (%s)" % (waves, how)},
+                   "parallelism": "1 GPU; workgroup = %s, %d wavefronts (%s)"
+                                  % ("rule x 64 scenarios" if mapping == 1 else
+                                     "one (rule, scenario) simulation", waves, how)},
         "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:

@@ -113,24 +113,36 @@ sigp_status sigp_batch_run(sigp_batch *batch, const double *sampled_dev, int64_t
 sigp_status sigp_batch_destroy(sigp_batch *batch);
 
 /* Launch mapping of the simulation kernel (no counterpart in the reference,
- * whose kernels always run one thread per (rule, scenario) pair).  A
- * workgroup simulates one rule on a block of 64 scenarios (one scenario per
- * lane); waves_per_item wavefronts (1, 2 or 4) split each decision's
- * candidates between them.  Every mapping gives bit-identical results.
- * Each wavefront keeps the candidate's terminal values and the postfix stack
+ * whose kernels always run one thread per (rule, scenario) pair).  Every
+ * mapping gives bit-identical results.
+ *   SIGP_MAP_SCENARIOS:  a workgroup simulates one rule on a block of 64
+ *     scenarios, one scenario per lane; its waves_per_item wavefronts split
+ *     each decision's candidates between them.
+ *   SIGP_MAP_CANDIDATES: a workgroup simulates one (rule, scenario) pair;
+ *     its 64 * waves_per_item threads split each decision's candidates and
+ *     take the first maximum together.  For few scenarios per rule, such as
+ *     the reference's GP loop, which evaluates one per generation
+ *     (engine/gp.hpp:32).
+ *   SIGP_MAP_AUTO: candidate lanes when T < 16, scenario lanes otherwise.
+ * Each wavefront keeps the candidates' terminal values and the postfix stack
  * in LDS, (distinct slots read + deepest stack - 1) * 512 bytes, and a
  * workgroup stays within 64 KiB of LDS: a batch whose rules need more runs
  * with fewer wavefronts than requested (autotune skips such mappings). */
+#define SIGP_MAP_AUTO 0
+#define SIGP_MAP_SCENARIOS 1
+#define SIGP_MAP_CANDIDATES 2
 typedef struct {
-    int32_t waves_per_item;   /* 1, 2 or 4; 0 = library default */
+    int32_t waves_per_item;   /* 1, 2 or 4; 0 = default of the mapping */
+    int32_t mapping;          /* SIGP_MAP_* */
 } sigp_launch_config;
 
 /* Mapping of one batch. */
 sigp_status sigp_batch_configure(sigp_batch *batch, const sigp_launch_config *cfg);
 
-/* Runs the batch once per candidate mapping on `stream` (device pointers as
- * in sigp_batch_run), keeps the fastest, and reports it in cfg_out
- * (nullable).  Synchronous. */
+/* Runs the batch twice per candidate configuration (both mappings, 1, 2 and
+ * 4 wavefronts where they fit) on `stream` (device pointers as in
+ * sigp_batch_run), keeps the fastest by the second run, and reports it in
+ * cfg_out (nullable).  Synchronous. */
 sigp_status sigp_batch_autotune(sigp_batch *batch, const double *sampled_dev, int64_t n_rows,
                                 void *stream, sigp_launch_config *cfg_out);
 

@@ -27,8 +27,9 @@ class OrcInstance(C.Structure):
 
 def build(reference: bool = True) -> None:
     """Compile the oracle (and, where the reference sources exist, _ref: the
-    reference's CPU simulator and its GPU gate linked against libsigp.so)."""
-    targets = ["liboracle.so", "liboracle_count.so"] + (["ref", "gate"] if reference else [])
+    reference's CPU simulator, and its GPU gate and tools linked against
+    libsigp.so)."""
+    targets = ["liboracle.so", "liboracle_count.so"] + (["ref", "gate", "tools"] if reference else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 

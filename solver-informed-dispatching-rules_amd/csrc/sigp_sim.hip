@@ -223,7 +223,7 @@ struct RuleSet {
     cip ops, args, off, len;
     cup rd;      // per rule: slots the rule reads (its leaves index them compactly)
     cup flags;   // per rule: fused computations (F_*) and REACH path lengths
-    cip order;   // launch order of (rule, scenario block) items, or null
+    cip order;   // launch order of the rules (longest first), or null = index order
     int P;
     int KL;      // value-store rows per wavefront: max over rules of popcount(rd), >= 1
     int SL;      // stack rows per wavefront: max postfix depth - 1, >= 1
@@ -291,7 +291,7 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
 // RARE = false compiles out the slots no scan family produces (44, 49, 50,
 // 53-57, 59, 60) for batches whose rules read none of them.
 template <int NW, bool RMOD, bool RARE>
-__device__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
+__device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
                       uint64_t rd, uint64_t fl, int cur, double ct, int rem,
                       const uint32_t (&vis)[NW], int i0, int istep, double *tv, double *st,
                       double &best_pri_out) {
@@ -428,12 +428,12 @@ void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
     __shared__ double lpri[4][64];
     __shared__ int lidx[4][64];
     extern __shared__ double lds_rows[];
-    const int item = R.order ? R.order[blockIdx.x] : (int)blockIdx.x;
+    const int rank = (int)blockIdx.x / S.nblk;
+    const int blk = (int)blockIdx.x - rank * S.nblk;
+    const int p = R.order ? R.order[rank] : rank;
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const int p = item / S.nblk;
-    const int blk = item - p * S.nblk;
     const int s = blk * 64 + lane;
     const bool active = s < S.T;
     const int off = R.off[p], L = R.len[p];
@@ -494,6 +494,94 @@ void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
     if (active && wv == 0) out[(size_t)p * S.T + s] = total;
 }
 
+// Reduction step of the argmax below: does (pb, ib) beat (pa, ia)?  Index -1
+// marks "no candidate"; a candidate's priority is always above -1e300.
+__device__ __forceinline__ bool beats(double pb, int ib, double pa, int ia) {
+    return ib >= 0 && (ia < 0 || pb > pa || (pb == pa && ib < ia));
+}
+
+// Workgroup-wide first maximum: the largest priority, the lowest candidate
+// index among equals -- the reference's strict ">" scan in index order
+// (engine/sim_core.hpp:199-203).  Every thread gets the winner (or -1).
+__device__ __forceinline__ int block_argmax(double pri, int idx, int W, int wv, int lane,
+                                            double *rpri, int *ridx) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double op = __shfl_xor(pri, off);
+        const int oi = __shfl_xor(idx, off);
+        if (beats(op, oi, pri, idx)) { pri = op; idx = oi; }
+    }
+    if (W > 1) {
+        if (lane == 0) { rpri[wv] = pri; ridx[wv] = idx; }
+        __syncthreads();
+        pri = rpri[0];
+        idx = ridx[0];
+        for (int w = 1; w < W; ++w)
+            if (beats(rpri[w], ridx[w], pri, idx)) { pri = rpri[w]; idx = ridx[w]; }
+        __syncthreads();
+    }
+    return idx;
+}
+
+// Candidate-lane mapping: one workgroup = one (rule, scenario) simulation
+// (engine/sim_core.hpp:207-238); thread t evaluates candidates t, t + 64W,
+// ... of every decision and the workgroup takes the first maximum.  Route
+// state is identical in every thread.  Suited to few scenarios per rule (the
+// reference's GP loop evaluates one per generation, engine/gp.hpp:32), where
+// scenario lanes would sit idle.  Dynamic LDS as in sim_kernel.
+template <int NW, bool RMOD, bool RARE>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 256)))
+void sim_cand_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S, double *__restrict__ out) {
+    __shared__ double rpri[4];
+    __shared__ int ridx[4];
+    extern __shared__ double lds_rows[];
+    const int rank = (int)blockIdx.x / S.T;
+    const int s = (int)blockIdx.x - rank * S.T;
+    const int p = R.order ? R.order[rank] : rank;
+    const int W = (int)(blockDim.x >> 6);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const int off = R.off[p], L = R.len[p];
+    const cip ops = R.ops + off;
+    const cip args = R.args + 2 * (size_t)off;
+    const uint64_t rd = R.rd[p];
+    const uint64_t fl = R.flags[p];
+    const size_t row = (size_t)(p / S.group_size) * S.T + s;
+    const double *__restrict__ samp = S.sampled + row * (size_t)I.n;
+    double *tv = lds_rows + (size_t)wv * (R.KL + R.SL) * 64 + lane;
+    double *st = tv + (size_t)R.KL * 64;
+
+    uint32_t vis[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) vis[w] = 0;
+    int cur = 0, veh = 0;
+    double ct = 0.0, total = 0.0;
+    bool running = I.m > 0;
+    while (running) {
+        double bp = -1e300;
+        const int rem = I.m - veh - 1;
+        int best = decide<NW, RMOD, RARE>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis,
+                                          (int)threadIdx.x, (int)blockDim.x, tv, st, bp);
+        best = block_argmax(bp, best, W, wv, lane, rpri, ridx);
+        if (best < 0) {
+            ++veh;
+            cur = 0;
+            ct = 0.0;
+            if (veh >= I.m) running = false;
+        } else {
+            const int pb = best + 1;
+            const double ta = I.dist[(size_t)cur * I.ld + pb];
+            const double a = ct + ta;
+            const double stt = a > I.open[best] ? a : I.open[best];
+            ct = stt + samp[best];
+            cur = pb;
+            total += I.score[best];
+            wset<NW>(vis, best);
+        }
+    }
+    if (threadIdx.x == 0) out[(size_t)p * S.T + s] = total;
+}
+
 // Per-rule mean in scenario order, then / T (engine/gp_engine.cu:76-81).
 __global__ void mean_kernel(const double *__restrict__ pairs, int P, int T, double *__restrict__ mean) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -538,18 +626,19 @@ struct sigp_instance {
     size_t out_cap = 0;
     double *mean_buf = nullptr;
     size_t mean_cap = 0;
-    int waves_per_item = 0;         // 0 = choose per batch
+    int waves_per_item = 0;         // default launch mapping of new batches (0 = per batch)
+    int mapping = SIGP_MAP_AUTO;
 };
 
 // Two cooperating wavefronts per block once there are enough blocks to fill
 // the GPU twice over (256 CUs x 8 resident waves); four below that, so a
 // small batch still spreads its candidates over more of the chip.
-static int default_waves(long long items) { return items >= 2048 ? 2 : 4; }
 
 struct sigp_batch {
     sigp_instance *inst = nullptr;
     int P = 0, T = 0, group_size = 1;
-    int waves_per_item = 4;         // cooperating wavefronts per (rule, scenario block)
+    int waves_per_item = 0;         // wavefronts per workgroup; 0 = default of the mapping
+    int mapping = SIGP_MAP_AUTO;    // SIGP_MAP_*; AUTO resolves from T at launch
     int64_t rows_needed = 0;
     bool has_rmod = false;          // some rule has an rmod node
     bool has_rare = false;          // some rule reads a slot outside the scan families
@@ -958,18 +1047,18 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
             if (comp[p] & RARE_SLOTS) c += 4.0f;
             cost[p] = c;
         }
-        order.resize((size_t)P * nblk);
-        for (size_t k = 0; k < order.size(); ++k) order[k] = (int32_t)k;
+        order.resize((size_t)P);
+        for (int k = 0; k < P; ++k) order[k] = k;
         std::stable_sort(order.begin(), order.end(),
-                         [&](int32_t a, int32_t b) { return cost[a / nblk] > cost[b / nblk]; });
+                         [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
     }
     sigp_batch *bt = new sigp_batch();
     bt->inst = inst;
     bt->P = P;
     bt->T = T;
     bt->group_size = group_size;
-    bt->waves_per_item = inst->waves_per_item > 0 ? inst->waves_per_item
-                                                   : default_waves((long long)P * ((T + 63) / 64));
+    bt->waves_per_item = inst->waves_per_item;
+    bt->mapping = inst->mapping;
     bt->rows_needed = P > 0 ? (int64_t)((P - 1) / group_size + 1) * T : 0;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes + 8); return o; };
@@ -1016,21 +1105,31 @@ extern "C" sigp_status sigp_batch_destroy(sigp_batch *bt) {
     return SIGP_OK;
 }
 
+static sigp_status check_config(const sigp_launch_config *cfg) {
+    const int w = cfg->waves_per_item;
+    if (w != 0 && w != 1 && w != 2 && w != 4)
+        return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
+    if (cfg->mapping != SIGP_MAP_AUTO && cfg->mapping != SIGP_MAP_SCENARIOS &&
+        cfg->mapping != SIGP_MAP_CANDIDATES)
+        return fail(SIGP_INVALID_ARG, "mapping must be SIGP_MAP_AUTO, _SCENARIOS or _CANDIDATES");
+    return SIGP_OK;
+}
+
 extern "C" sigp_status sigp_batch_configure(sigp_batch *bt, const sigp_launch_config *cfg) {
     if (!bt || !cfg) return fail(SIGP_INVALID_ARG, "batch and cfg must be non-NULL");
-    const int w = cfg->waves_per_item;
-    if (w == 0) bt->waves_per_item = default_waves((long long)bt->P * ((bt->T + 63) / 64));
-    else if (w == 1 || w == 2 || w == 4) bt->waves_per_item = w;
-    else return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
+    const sigp_status st = check_config(cfg);
+    if (st != SIGP_OK) return st;
+    bt->waves_per_item = cfg->waves_per_item;
+    bt->mapping = cfg->mapping;
     return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_launch_config *cfg) {
     if (!inst || !cfg) return fail(SIGP_INVALID_ARG, "inst and cfg must be non-NULL");
-    const int w = cfg->waves_per_item;
-    if (w != 0 && w != 1 && w != 2 && w != 4)
-        return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
-    inst->waves_per_item = w;
+    const sigp_status st = check_config(cfg);
+    if (st != SIGP_OK) return st;
+    inst->waves_per_item = cfg->waves_per_item;
+    inst->mapping = cfg->mapping;
     return SIGP_OK;
 }
 
@@ -1052,23 +1151,57 @@ static int fit_waves(const sigp_batch *bt, int w) {
     return w;
 }
 
+// AUTO: candidate lanes below 16 scenarios per rule (scenario lanes would be
+// mostly idle), scenario lanes otherwise.
+static int resolved_mapping(const sigp_batch *bt) {
+    if (bt->mapping != SIGP_MAP_AUTO) return bt->mapping;
+    return bt->T < 16 ? SIGP_MAP_CANDIDATES : SIGP_MAP_SCENARIOS;
+}
+
+// Wavefronts per workgroup: the configured count, else 4 for scenario lanes
+// and enough 64-candidate groups to cover n (at most 4) for candidate lanes;
+// then reduced until the workgroup's LDS fits.
+static int resolved_waves(const sigp_batch *bt, int mapping) {
+    int w = bt->waves_per_item;
+    if (w == 0) {
+        w = 4;
+        if (mapping == SIGP_MAP_CANDIDATES) {
+            const int need = (bt->inst->n + 63) / 64;
+            w = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
+        }
+    }
+    return fit_waves(bt, w);
+}
+
+template <int NW, bool RMOD, bool RARE>
+static void launch_variant(const sigp_batch *bt, int mapping, int w, const ScenSpec &S, double *out,
+                           hipStream_t stream) {
+    const sigp_instance *I = bt->inst;
+    const unsigned lds = (unsigned)(w * lds_per_wave(bt));
+    if (mapping == SIGP_MAP_CANDIDATES) {
+        const unsigned items = (unsigned)((long long)bt->P * S.T);
+        sim_cand_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+    } else {
+        const unsigned items = (unsigned)((long long)bt->P * S.nblk);
+        sim_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+    }
+}
+
 template <int NW>
 static void launch_sim(const sigp_batch *bt, const double *sampled, double *out, hipStream_t stream) {
-    const sigp_instance *I = bt->inst;
     ScenSpec S;
     S.sampled = sampled;
     S.T = bt->T;
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
-    const unsigned items = (unsigned)((long long)bt->P * S.nblk);
-    const int w = fit_waves(bt, bt->waves_per_item);
-    const unsigned lds = (unsigned)(w * lds_per_wave(bt));
+    const int mapping = resolved_mapping(bt);
+    const int w = resolved_waves(bt, mapping);
     if (bt->has_rmod) {
-        if (bt->has_rare) sim_kernel<NW, true, true><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
-        else sim_kernel<NW, true, false><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+        if (bt->has_rare) launch_variant<NW, true, true>(bt, mapping, w, S, out, stream);
+        else launch_variant<NW, true, false>(bt, mapping, w, S, out, stream);
     } else {
-        if (bt->has_rare) sim_kernel<NW, false, true><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
-        else sim_kernel<NW, false, false><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
+        if (bt->has_rare) launch_variant<NW, false, true>(bt, mapping, w, S, out, stream);
+        else launch_variant<NW, false, false>(bt, mapping, w, S, out, stream);
     }
 }
 
@@ -1107,7 +1240,7 @@ extern "C" sigp_status sigp_batch_autotune(sigp_batch *bt, const double *sampled
                                            void *stream, sigp_launch_config *cfg_out) {
     if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
     if (bt->P == 0 || bt->T == 0) {
-        if (cfg_out) cfg_out->waves_per_item = bt->waves_per_item;
+        if (cfg_out) { cfg_out->waves_per_item = bt->waves_per_item; cfg_out->mapping = bt->mapping; }
         return SIGP_OK;
     }
     if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->P * bt->T * 8));
@@ -1115,27 +1248,42 @@ extern "C" sigp_status sigp_batch_autotune(sigp_batch *bt, const double *sampled
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const int cand[3] = {1, 2, 4};
-    int best_w = bt->waves_per_item;
+    const int ncand_groups = (bt->inst->n + 63) / 64;
+    struct Cand { int mapping, w; };
+    const Cand cands[6] = {{SIGP_MAP_SCENARIOS, 1}, {SIGP_MAP_SCENARIOS, 2}, {SIGP_MAP_SCENARIOS, 4},
+                           {SIGP_MAP_CANDIDATES, 1}, {SIGP_MAP_CANDIDATES, 2}, {SIGP_MAP_CANDIDATES, 4}};
+    const int keep_w = bt->waves_per_item, keep_m = bt->mapping;
+    int best_w = 0, best_m = SIGP_MAP_AUTO;
     float best_ms = 0.0f;
     sigp_status st = SIGP_OK;
-    for (int k = 0; k < 3 && st == SIGP_OK; ++k) {
-        if (fit_waves(bt, cand[k]) != cand[k]) break;   // larger mappings do not fit LDS
-        bt->waves_per_item = cand[k];
-        (void)hipEventRecord(e0, s);
-        st = sigp_batch_run(bt, sampled_dev, n_rows, bt->pairs_tmp, nullptr, stream);
-        (void)hipEventRecord(e1, s);
-        if (st != SIGP_OK) break;
-        if (hipEventSynchronize(e1) != hipSuccess) { st = fail(SIGP_DEVICE_ERROR, "autotune run failed"); break; }
+    for (const Cand &c : cands) {
+        if (fit_waves(bt, c.w) != c.w) continue;                         // LDS does not fit
+        if (c.mapping == SIGP_MAP_CANDIDATES && c.w > 1 && 32 * c.w >= ncand_groups * 64) continue;
+        bt->waves_per_item = c.w;
+        bt->mapping = c.mapping;
         float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (k == 0 || ms < best_ms) { best_ms = ms; best_w = cand[k]; }
+        for (int rep = 0; rep < 2 && st == SIGP_OK; ++rep) {   // the first run loads the code object
+            (void)hipEventRecord(e0, s);
+            st = sigp_batch_run(bt, sampled_dev, n_rows, bt->pairs_tmp, nullptr, stream);
+            (void)hipEventRecord(e1, s);
+            if (st != SIGP_OK) break;
+            if (hipEventSynchronize(e1) != hipSuccess) { st = fail(SIGP_DEVICE_ERROR, "autotune run failed"); break; }
+            (void)hipEventElapsedTime(&ms, e0, e1);
+        }
+        if (st != SIGP_OK) break;
+        if (best_w == 0 || ms < best_ms) { best_ms = ms; best_w = c.w; best_m = c.mapping; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (st != SIGP_OK || best_w == 0) {
+        bt->waves_per_item = keep_w;
+        bt->mapping = keep_m;
+        return st;
+    }
     bt->waves_per_item = best_w;
-    if (cfg_out) cfg_out->waves_per_item = best_w;
-    return st;
+    bt->mapping = best_m;
+    if (cfg_out) { cfg_out->waves_per_item = best_w; cfg_out->mapping = best_m; }
+    return SIGP_OK;
 }
 
 static sigp_status ensure(double **buf, size_t *cap, size_t bytes) {

@@ -8,7 +8,8 @@ import ctypes as C
 import os
 from typing import Optional
 
-__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "EXPORTED"]
+__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "EXPORTED",
+           "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libsigp.so")
@@ -36,8 +37,11 @@ class InstanceDesc(C.Structure):
                 ("zp", C.c_void_p)]
 
 
+MAP_AUTO, MAP_SCENARIOS, MAP_CANDIDATES = 0, 1, 2
+
+
 class LaunchConfig(C.Structure):
-    _fields_ = [("waves_per_item", C.c_int32)]
+    _fields_ = [("waves_per_item", C.c_int32), ("mapping", C.c_int32)]
 
 
 class Rules(C.Structure):

@@ -10,11 +10,12 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import InstanceDesc, LaunchConfig, Rules, check, lib
+from ._lib import MAP_AUTO, MAP_CANDIDATES, MAP_SCENARIOS, InstanceDesc, LaunchConfig, Rules, check, lib
 from .instance import Instance
 from .rules import N_TERMS, Packed
 
-__all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info"]
+__all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info", "MAP_AUTO",
+           "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -100,17 +101,19 @@ class Batch:
                                    None if out_mean_ptr is None else C.c_void_p(out_mean_ptr),
                                    None if stream is None else C.c_void_p(stream)))
 
-    def configure(self, waves_per_item: int) -> None:
-        cfg = LaunchConfig(waves_per_item)
+    def configure(self, waves_per_item: int, mapping: int = MAP_AUTO) -> None:
+        """Launch mapping: MAP_SCENARIOS / MAP_CANDIDATES / MAP_AUTO and the
+        wavefronts per workgroup (1, 2, 4; 0 = the mapping's default)."""
+        cfg = LaunchConfig(waves_per_item, mapping)
         check(lib().sigp_batch_configure(self._h, C.byref(cfg)))
 
-    def autotune(self, sampled_ptr: int, n_rows: int, stream: Optional[int] = None) -> int:
-        """Time every mapping, keep the fastest; returns its waves_per_item."""
-        cfg = LaunchConfig(0)
+    def autotune(self, sampled_ptr: int, n_rows: int, stream: Optional[int] = None):
+        """Time every mapping, keep the fastest; returns (waves_per_item, mapping)."""
+        cfg = LaunchConfig(0, 0)
         check(lib().sigp_batch_autotune(self._h, C.c_void_p(sampled_ptr), n_rows,
                                         None if stream is None else C.c_void_p(stream),
                                         C.byref(cfg)))
-        return cfg.waves_per_item
+        return cfg.waves_per_item, cfg.mapping
 
     def close(self) -> None:
         if self._h:
@@ -135,9 +138,9 @@ class FactoryContext:
         self._h = C.c_void_p()
         check(lib().sigp_instance_create(C.byref(d), C.byref(self._h)))
 
-    def configure(self, waves_per_item: int) -> None:
-        """Default mapping of later batches and evaluate* calls (0 = automatic)."""
-        cfg = LaunchConfig(waves_per_item)
+    def configure(self, waves_per_item: int, mapping: int = MAP_AUTO) -> None:
+        """Default launch mapping of later batches and evaluate* calls."""
+        cfg = LaunchConfig(waves_per_item, mapping)
         check(lib().sigp_instance_configure(self._h, C.byref(cfg)))
 
     def close(self) -> None:

@@ -17,7 +17,8 @@ from .scenarios import gen_durations
 
 TEST_SEED = 1_000_000   # engine/gp_engine.cu:22
 
-__all__ = ["Workload", "plain_small", "large", "random_rules", "TEST_SEED"]
+__all__ = ["Workload", "plain_small", "large", "gp_generation", "gp_reference_generation",
+           "random_rules", "TEST_SEED"]
 
 
 @dataclass
@@ -27,8 +28,9 @@ class Workload:
     trees: List[Node]
     packed: Packed
     mask: np.ndarray
-    sampled: np.ndarray      # S x n
-    n_scen: int
+    sampled: np.ndarray      # rows of n durations
+    n_scen: int              # scenarios per rule (T)
+    group_size: int = 0      # rule p reads rows (p // group_size) * T + s; 0 = one group
 
     @property
     def n_sims(self) -> int:
@@ -60,6 +62,30 @@ def large(n_rules: int = 4096, n_scen: int = 2000, n: int = 288, m: int = 4) -> 
     sampled = gen_durations(inst.durations, n_scen, TEST_SEED, mode="gamma", shape=2.0)
     return Workload("large n=%d m=%d %dx%d" % (n, m, n_rules, n_scen), inst, trees,
                     pack_trees(trees), term_mask_from(EXTRA), sampled, n_scen)
+
+
+def gp_generation(pop: int = 1024, n_scen: int = 50) -> Workload:
+    """One generation of the on-device GP loop: the clustered instance
+    (n=100, m=3, windows U[30,120]), a ramped half-and-half population over
+    NS-11 + EXTRA-18 (max depth 8), 50 common training scenarios (Gamma 4)."""
+    inst = clustered_gp_instance()
+    trees = random_rules(pop, NS_TERMINALS + EXTRA, seed=0, min_depth=2, max_depth=8)
+    sampled = gen_durations(inst.durations, n_scen, TEST_SEED, mode="gamma", shape=4.0)
+    return Workload("gp-generation n=100 m=3 %dx%d" % (pop, n_scen), inst, trees,
+                    pack_trees(trees), ext29_mask(), sampled, n_scen)
+
+
+def gp_reference_generation(seeds: int = 30, pop: int = 1024) -> Workload:
+    """One launch of the reference's GP loop (engine/gp_engine.cu:155-164):
+    `seeds` independent populations, one training scenario each (Mei
+    protocol, engine/gp.hpp:32), rule p reading scenario row p // pop.
+    n_scen is 1; the scenario rows are one per seed."""
+    inst = clustered_gp_instance()
+    trees = random_rules(seeds * pop, NS_TERMINALS + EXTRA, seed=0, min_depth=2, max_depth=8)
+    sampled = np.concatenate([gen_durations(inst.durations, 1, s * 10000, mode="gamma", shape=4.0)
+                              for s in range(seeds)])
+    return Workload("gp-reference-generation n=100 m=3 %d seeds x %d rules x 1" % (seeds, pop),
+                    inst, trees, pack_trees(trees), ext29_mask(), sampled, 1, pop)
 
 
 def ext29_mask() -> np.ndarray:

@@ -31,3 +31,14 @@ def test_library_exports_every_symbol():
 def test_last_error_is_empty_string_initially():
     from sigp._lib import lib
     assert lib().sigp_last_error() == b""
+
+
+def test_library_is_newer_than_its_sources():
+    """libsigp.so was rebuilt after the last change to any source it is
+    compiled from (a stale library would be tested and benchmarked)."""
+    from sigp._lib import lib_path
+    csrc = os.path.join(ROOT, "solver-informed-dispatching-rules_amd", "csrc")
+    srcs = [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith((".hip", ".h"))]
+    srcs.append(HEADER)
+    newest = max(os.path.getmtime(f) for f in srcs)
+    assert os.path.getmtime(lib_path) >= newest, "libsigp.so is older than its sources: rebuild"
