@@ -98,6 +98,11 @@ def describe(kind, wl):
     if kind == "gp-generation":
         return ("one GP generation: clustered instance n=100 m=3, %d rules (ramped half-and-half, "
                 "depth 3-9 levels, NS-11 + EXTRA-18) x %d Gamma(4) scenarios" % (P, T))
+    if kind == "batched":
+        ns = sorted(i.n for i in wl.insts)
+        return ("batched uneven: %d instances (n %d-%d, m 1-4, uniform/clustered, windows U[10,200]) "
+                "x %d rules (up to 127 nodes, EXTRA-18) x %d Gamma(4) scenarios each, one multi-instance "
+                "batch" % (len(wl.insts), ns[0], ns[-1], wl.packed.n_rules, wl.n_scen))
     return ("one launch of the reference GP loop: clustered instance n=100 m=3, %d seeds x %d rules "
             "(NS-11 + EXTRA-18, depth 3-9 levels) x 1 scenario each" % (P // wl.group_size,
                                                                       wl.group_size))
@@ -110,11 +115,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rules", type=int, default=1024)
     ap.add_argument("--scenarios", type=int, default=500)
-    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference"], default="plain",
+    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched"],
+                    default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
                          "gp-reference: one launch of the reference's GP loop (30 seeds x 1,024 "
-                         "rules x 1 scenario)")
+                         "rules x 1 scenario); batched: 40 uneven instances (n 48-288) x 256 "
+                         "rules x 500 scenarios in one multi-instance batch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--waves", type=int, default=0,
@@ -130,25 +137,37 @@ def main():
         sys.exit(2)
 
     import torch
-    from sigp.api import FactoryContext, device_info
-    from sigp.workloads import gp_generation, gp_reference_generation, plain_small
+    from sigp.api import FactoryContext, batch_multi, device_info
+    from sigp.workloads import batched_uneven, gp_generation, gp_reference_generation, plain_small
 
     t0 = time.perf_counter()
     if args.workload == "plain":
         wl = plain_small(n_rules=args.rules, n_scen=args.scenarios)
     elif args.workload == "gp-generation":
         wl = gp_generation()
-    else:
+    elif args.workload == "gp-reference":
         wl = gp_reference_generation()
+    else:
+        wl = batched_uneven()
     log("workload built in %.1fs: %s" % (time.perf_counter() - t0, wl.name))
     dev = torch.device("cuda:0")
     name, n_cu, clk = device_info()
     log("device", name, n_cu, "CUs")
-    ctx = FactoryContext(wl.inst)
-    P, T = wl.packed.n_rules, wl.n_scen
-    batch = ctx.batch(wl.packed, T, wl.mask, group_size=wl.group_size or None)
-    n_rows = wl.sampled.shape[0]
-    sampled = torch.from_numpy(wl.sampled).to(dev)
+    T = wl.n_scen
+    if args.workload == "batched":
+        ctxs = [FactoryContext(i) for i in wl.insts]
+        ctx = ctxs[0]
+        batch = batch_multi(ctxs, wl.packed, T, wl.mask)
+        n_rows = len(wl.insts) * T
+        host_rows = np.concatenate([b.ravel() for b in wl.blocks])
+        P = len(wl.insts) * wl.packed.n_rules     # (instance, rule) pairs
+    else:
+        ctx = FactoryContext(wl.inst)
+        batch = ctx.batch(wl.packed, T, wl.mask, group_size=wl.group_size or None)
+        n_rows = wl.sampled.shape[0]
+        host_rows = wl.sampled
+        P = wl.packed.n_rules
+    sampled = torch.from_numpy(host_rows).to(dev)
     pairs = torch.zeros(P * T, dtype=torch.float64, device=dev)
     means = torch.zeros(P, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev)
@@ -161,6 +180,7 @@ def main():
         batch.configure(args.waves, mapping)
         waves, how = args.waves, "fixed"
     else:
+        log("autotuning the launch mapping ...")
         (waves, mapping), how = batch.autotune(sampled.data_ptr(), n_rows, stream.cuda_stream), "autotuned"
     lanes = "64-scenario block" if mapping == 1 else "(rule, scenario) simulation"
     log("%s mapping: %d wavefronts per %s" % (how, waves, lanes))
@@ -188,7 +208,8 @@ def main():
     assert np.all(np.isfinite(pr)) and np.all(pr == np.round(pr))
 
     roof = None
-    model = None if args.no_roofline else work_model(wl)
+    # the work model and the reference's CPU timing exist for the headline workload
+    model = None if (args.no_roofline or args.workload != "plain") else work_model(wl)
     if model is not None:
         # The simulation kernel alone (no per-rule mean kernel), same stream.
         kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -219,7 +240,7 @@ def main():
         del src, dst
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, ctx.zp)
+        cpu = cpu_baseline(wl, ctx.zp) if args.workload == "plain" else None
         log("cpu baseline", cpu)
     out = {
         "metric": "rule x scenario sims/sec", "value": round(value, 1), "unit": "sims/s",
@@ -227,7 +248,9 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": describe(args.workload, wl),
-                   "rules": P, "scenarios": T, "n": wl.inst.n, "m": wl.inst.m,
+                   "rules": P, "scenarios": T,
+                   "n": wl.inst.n if hasattr(wl, "inst") else None,
+                   "m": wl.inst.m if hasattr(wl, "inst") else None,
                    "parallelism": "1 GPU; workgroup = %s, %d wavefronts (%s)"
                                   % ("rule x 64 scenarios" if mapping == 1 else
                                      "one (rule, scenario) simulation", waves, how)},

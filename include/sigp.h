@@ -112,6 +112,29 @@ sigp_status sigp_batch_run(sigp_batch *batch, const double *sampled_dev, int64_t
                            double *out_pairs_dev, double *out_mean_dev, void *stream);
 sigp_status sigp_batch_destroy(sigp_batch *batch);
 
+/* Many instances, the same rules (no counterpart in the reference, which
+ * evaluates one instance per FactoryContext and launch; its GA and
+ * redeploy tools loop over instances, s4_selection/redeploy.cu:40).  Every
+ * rule is simulated on every instance over that instance's own T scenario
+ * rows.  Instances of different sizes run in one launch per size class
+ * (n <= 128, <= 288, <= 512), each class with its own longest-first order
+ * and, after sigp_batch_autotune, its own mapping.  The instances must
+ * outlive the batch.  With sigp_batch_run, sampled_dev holds the instances'
+ * row blocks back to back (instance k: T rows of its n_k doubles), n_rows is
+ * their total row count (n_inst * T), out_pairs receives n_inst * P * T
+ * rewards, out[(k * P + p) * T + s], and out_mean n_inst * P means, k * P + p.
+ * Rule p reads mask row p / mask_group. */
+sigp_status sigp_batch_create_multi(sigp_instance *const *insts, int32_t n_inst,
+                                    const sigp_rules *rules, int32_t T, const int64_t *masks,
+                                    int32_t n_masks, int32_t mask_group, sigp_batch **out);
+
+/* sigp_batch_create_multi + sigp_batch_run on host arrays, synchronous:
+ * `sampled` holds the instances' row blocks back to back as above. */
+sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst, const sigp_rules *rules,
+                                const double *sampled, int32_t T, const int64_t *masks,
+                                int32_t n_masks, int32_t mask_group, double *out_pairs,
+                                double *out_mean);
+
 /* Launch mapping of the simulation kernel (no counterpart in the reference,
  * whose kernels always run one thread per (rule, scenario) pair).  Every
  * mapping gives bit-identical results.
@@ -139,10 +162,13 @@ typedef struct {
 /* Mapping of one batch. */
 sigp_status sigp_batch_configure(sigp_batch *batch, const sigp_launch_config *cfg);
 
-/* Runs the batch twice per candidate configuration (both mappings, 1, 2 and
- * 4 wavefronts where they fit) on `stream` (device pointers as in
- * sigp_batch_run), keeps the fastest by the second run, and reports it in
- * cfg_out (nullable).  Synchronous. */
+/* Times every candidate configuration (both mappings, 1, 2 and 4 wavefronts
+ * where they fit), twice each, on `stream` (device pointers as in
+ * sigp_batch_run) and keeps the fastest by the second run, per size class.
+ * Each class is timed on an evenly strided sample of its (instance, rule)
+ * pairs, about 2^19 simulations, so tuning costs a fraction of a run of a
+ * large batch.  cfg_out (nullable) receives the configuration of the class
+ * with the most work.  Synchronous; the results it writes are scratch. */
 sigp_status sigp_batch_autotune(sigp_batch *batch, const double *sampled_dev, int64_t n_rows,
                                 void *stream, sigp_launch_config *cfg_out);
 

@@ -223,18 +223,31 @@ struct RuleSet {
     cip ops, args, off, len;
     cup rd;      // per rule: slots the rule reads (its leaves index them compactly)
     cup flags;   // per rule: fused computations (F_*) and REACH path lengths
-    cip order;   // launch order of the rules (longest first), or null = index order
+    cip order;   // launch order of (instance, rule) pairs k * P + p, longest first; null = k = 0, p = index
     int P;
     int KL;      // value-store rows per wavefront: max over rules of popcount(rd), >= 1
     int SL;      // stack rows per wavefront: max postfix depth - 1, >= 1
 };
 
 struct ScenSpec {
-    const double *sampled;   // rows of n durations
+    const double *sampled;   // rows of n durations (instance k's rows start at its samp_off)
     int T;                   // scenarios per rule
-    int group_size;          // rule p reads rows (p / group_size) * T + s
+    int group_size;          // rule p reads rows (p / group_size) * T + s of its instance
     int nblk;                // ceil(T / 64)
 };
+
+// One instance of a launch: its tables and where its scenario rows and
+// results start.  A single-instance batch has one slot; a multi-instance
+// batch one per instance, all evaluating the same rules.
+struct InstSlot {
+    DevInst dev;
+    FusedTabs tabs;
+    long long samp_off;   // first double of the instance's scenario rows
+    long long out_off;    // first result: instance k's pairs are out[out_off + p * T + s]
+};
+// Kernels take the slots as a read-only, non-aliased argument, so that their
+// wave-uniform reads of it are scalar loads.
+typedef const InstSlot *__restrict__ cslot;
 
 // Postfix interpreter (engine/sim_core.hpp:11-41).  Leaves read the
 // candidate's terminal values from the wave's LDS value store (`tv`, one
@@ -423,14 +436,17 @@ __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, cons
 template <int NW, bool RMOD, bool RARE>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 256),
                           amdgpu_waves_per_eu(1)))
-void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
-                                                  double *__restrict__ out) {
+void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     __shared__ double lpri[4][64];
     __shared__ int lidx[4][64];
     extern __shared__ double lds_rows[];
     const int rank = (int)blockIdx.x / S.nblk;
     const int blk = (int)blockIdx.x - rank * S.nblk;
-    const int p = R.order ? R.order[rank] : rank;
+    const int pair = R.order ? R.order[rank] : rank;
+    const int k = pair / R.P;
+    const int p = pair - k * R.P;
+    const DevInst I = slots[k].dev;
+    const FusedTabs F = slots[k].tabs;
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -442,7 +458,7 @@ void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
     const uint64_t rd = R.rd[p];
     const uint64_t fl = R.flags[p];
     const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
-    const double *__restrict__ samp = S.sampled + row * (size_t)I.n;
+    const double *__restrict__ samp = S.sampled + slots[k].samp_off + row * (size_t)I.n;
     double *tv = lds_rows + (size_t)wv * (R.KL + R.SL) * 64 + lane;
     double *st = tv + (size_t)R.KL * 64;
 
@@ -491,7 +507,7 @@ void sim_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S,
             }
         }
     }
-    if (active && wv == 0) out[(size_t)p * S.T + s] = total;
+    if (active && wv == 0) out[slots[k].out_off + (size_t)p * S.T + s] = total;
 }
 
 // Reduction step of the argmax below: does (pb, ib) beat (pa, ia)?  Index -1
@@ -531,13 +547,17 @@ __device__ __forceinline__ int block_argmax(double pri, int idx, int W, int wv, 
 // scenario lanes would sit idle.  Dynamic LDS as in sim_kernel.
 template <int NW, bool RMOD, bool RARE>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 256)))
-void sim_cand_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S, double *__restrict__ out) {
+void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     __shared__ double rpri[4];
     __shared__ int ridx[4];
     extern __shared__ double lds_rows[];
     const int rank = (int)blockIdx.x / S.T;
     const int s = (int)blockIdx.x - rank * S.T;
-    const int p = R.order ? R.order[rank] : rank;
+    const int pair = R.order ? R.order[rank] : rank;
+    const int k = pair / R.P;
+    const int p = pair - k * R.P;
+    const DevInst I = slots[k].dev;
+    const FusedTabs F = slots[k].tabs;
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -547,7 +567,7 @@ void sim_cand_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S, double *__re
     const uint64_t rd = R.rd[p];
     const uint64_t fl = R.flags[p];
     const size_t row = (size_t)(p / S.group_size) * S.T + s;
-    const double *__restrict__ samp = S.sampled + row * (size_t)I.n;
+    const double *__restrict__ samp = S.sampled + slots[k].samp_off + row * (size_t)I.n;
     double *tv = lds_rows + (size_t)wv * (R.KL + R.SL) * 64 + lane;
     double *st = tv + (size_t)R.KL * 64;
 
@@ -579,7 +599,7 @@ void sim_cand_kernel(DevInst I, FusedTabs F, RuleSet R, ScenSpec S, double *__re
             wset<NW>(vis, best);
         }
     }
-    if (threadIdx.x == 0) out[(size_t)p * S.T + s] = total;
+    if (threadIdx.x == 0) out[slots[k].out_off + (size_t)p * S.T + s] = total;
 }
 
 // Per-rule mean in scenario order, then / T (engine/gp_engine.cu:76-81).
@@ -630,21 +650,37 @@ struct sigp_instance {
     int mapping = SIGP_MAP_AUTO;
 };
 
-// Two cooperating wavefronts per block once there are enough blocks to fill
-// the GPU twice over (256 CUs x 8 resident waves); four below that, so a
-// small batch still spreads its candidates over more of the chip.
+// Launch order and tuned mapping of the instances of one size class (one
+// visited-set width NW of the kernels).
+struct LaunchUnit {
+    int nwb = 4;                    // NW of the class's kernels
+    int max_n = 0;                  // largest instance of the class
+    long long n_pairs = 0;          // (instance, rule) pairs of the class
+    cip order = nullptr;            // their launch order, pair = k * P + p, longest first
+    int mapping = SIGP_MAP_AUTO;    // tuned for this class; AUTO = follow the batch
+    int waves = 0;                  // tuned for this class; 0 = follow the batch
+    cip tune_order = nullptr;       // every s-th pair of `order`: the autotuner's sample
+    long long tune_pairs = 0;
+};
 
 struct sigp_batch {
-    sigp_instance *inst = nullptr;
+    std::vector<sigp_instance *> insts;   // one, or the instances of a multi-instance batch
+    bool multi = false;
     int P = 0, T = 0, group_size = 1;
     int waves_per_item = 0;         // wavefronts per workgroup; 0 = default of the mapping
     int mapping = SIGP_MAP_AUTO;    // SIGP_MAP_*; AUTO resolves from T at launch
-    int64_t rows_needed = 0;
+    int64_t rows_needed = 0;        // scenario rows sigp_batch_run needs
+    long long n_out = 0;            // results: instances x P x T
     bool has_rmod = false;          // some rule has an rmod node
     bool has_rare = false;          // some rule reads a slot outside the scan families
-    void *dbuf = nullptr;
-    RuleSet rs{};
+    void *dbuf = nullptr;           // rules, instance slots and launch orders
+    RuleSet rs{};                   // rule arrays (order: set per launch unit)
+    cslot slots = nullptr;
+    int n_units = 0;
+    LaunchUnit units[3];            // heaviest size class first
     double *pairs_tmp = nullptr;    // used when the caller wants means only
+    hipStream_t side[2] = {nullptr, nullptr};   // size classes 2 and 3 run here, concurrently
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
 };
 
 static sigp_status check_device(int *dev_out) {
@@ -945,28 +981,35 @@ static void compute_sets(uint64_t used, uint64_t mk, uint64_t *rd_out, uint64_t 
     *fl_out = f;
 }
 
-extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *rules, int32_t T,
-                                         int32_t group_size, const int64_t *masks, int32_t n_masks,
-                                         int32_t mask_group, sigp_batch **out) {
-    if (!out) return fail(SIGP_INVALID_ARG, "out is NULL");
-    *out = nullptr;
-    if (!inst || !rules) return fail(SIGP_INVALID_ARG, "inst and rules must be non-NULL");
+// Validated rules, rewritten for the device: dead leaves become the constant
+// 0, live leaves index the compact value store; per rule the read set, the
+// fused computations and a relative cost estimate.
+struct PreparedRules {
+    int P = 0;
+    std::vector<int32_t> ops, args, offs, lens;
+    std::vector<uint64_t> rd, flags;
+    std::vector<float> cost;
+    int max_depth = 1, max_k = 1;
+    bool any_rmod = false;
+    uint64_t any_rd = 0;
+};
+
+static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, int32_t n_masks,
+                                 int32_t mask_group, PreparedRules &pr) {
     const int P = rules->n_rules;
     if (P < 0) return fail(SIGP_INVALID_ARG, "n_rules < 0");
-    if (T < 0) return fail(SIGP_INVALID_ARG, "T < 0");
-    if (group_size < 1 || mask_group < 1)
-        return fail(SIGP_INVALID_ARG, "group_size and mask_group must be >= 1");
+    if (mask_group < 1) return fail(SIGP_INVALID_ARG, "mask_group must be >= 1");
     if (P > 0 && (!rules->ops || !rules->args || !rules->offsets || !rules->lengths))
         return fail(SIGP_INVALID_ARG, "rule arrays must be non-NULL");
     if (P > 0 && (!masks || n_masks < 1)) return fail(SIGP_INVALID_ARG, "masks missing");
     if ((long long)(P > 0 ? (P - 1) / mask_group : 0) >= (long long)std::max(n_masks, 1))
         return fail(SIGP_INVALID_ARG, "mask row (n_rules-1)/mask_group out of range");
-
-    std::vector<int32_t> ops, args, offs(P), lens(P);
-    std::vector<uint64_t> comp(P), rdv(P), flg(P);
-    int max_depth = 1, max_k = 1;
-    bool any_rmod = false;
-    uint64_t any_rd = 0;
+    pr.P = P;
+    pr.offs.resize(P);
+    pr.lens.resize(P);
+    pr.rd.resize(P);
+    pr.flags.resize(P);
+    pr.cost.resize(P);
     for (int p = 0; p < P; ++p) {
         const int64_t o = rules->offsets[p], L = rules->lengths[p];
         if (o < 0 || L < 1 || o + L > rules->n_nodes)
@@ -988,25 +1031,25 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
             } else if (op == -2) {
                 ++depth;
             } else if (op >= 0 && op <= 6) {
-                if (op == 6) any_rmod = true;
+                if (op == 6) pr.any_rmod = true;
                 if (depth < 2)
                     return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack underflow");
                 --depth;
             } else {
                 return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": bad opcode");
             }
-            max_depth = std::max(max_depth, depth);
+            pr.max_depth = std::max(pr.max_depth, depth);
         }
         if (depth != 1)
             return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": postfix does not reduce to one value");
-        if (max_depth > SIGP_MAX_STACK)
+        if (pr.max_depth > SIGP_MAX_STACK)
             return fail(SIGP_INVALID_ARG, "rule " + std::to_string(p) + ": stack deeper than " +
                                               std::to_string(SIGP_MAX_STACK));
-        uint64_t r = 0, c = 0, fb = 0;
-        compute_sets(used, mk, &r, &c, &fb);
-        max_k = std::max(max_k, __builtin_popcountll(r));
-        offs[p] = (int32_t)ops.size();
-        lens[p] = (int32_t)L;
+        uint64_t r = 0, c = 0, f = 0;
+        compute_sets(used, mk, &r, &c, &f);
+        pr.max_k = std::max(pr.max_k, __builtin_popcountll(r));
+        pr.offs[p] = (int32_t)pr.ops.size();
+        pr.lens[p] = (int32_t)L;
         for (int64_t k = 0; k < L; ++k) {
             int op = rules->ops[o + k];
             int a0 = rules->args[2 * (o + k)], a1 = rules->args[2 * (o + k) + 1];
@@ -1014,69 +1057,136 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
                 if (!(r & B(a0))) { op = -2; a0 = 0; a1 = 0; }   // dead slot reads 0.0
                 else a0 = __builtin_popcountll(r & (B(a0) - 1));    // compact value-store row
             }
-            ops.push_back(op);
-            args.push_back(a0);
-            args.push_back(a1);
+            pr.ops.push_back(op);
+            pr.args.push_back(a0);
+            pr.args.push_back(a1);
         }
-        comp[p] = c;
-        rdv[p] = r;
-        any_rd |= r;
-        flg[p] = fb;
+        pr.rd[p] = r;
+        pr.any_rd |= r;
+        pr.flags[p] = f;
+        // Relative cost per candidate from the scans the flags select (the
+        // per-neighbour weights were measured on MI355X): the launch order
+        // puts expensive rules first so that they do not form its tail.
+        float w = 1.0f + 0.05f * (float)L;
+        if (f & (F_NS | F_MAXN | F_REACH2)) w += 1.0f;
+        if (f & F_REACH2) w += 1.0f;
+        if (f & (F_NSROB | F_R2ROB | F_REACHKR)) w += 1.0f;
+        if (f & F_R2ROB) w += 1.0f;
+        if (f & F_REGRET) w += 0.5f;
+        if (f & F_STRUCT) w += 2.0f;
+        if (f & F_NSTW) w += 1.0f;
+        if (f & F_ATRISK) w += 8.0f;
+        if (f & F_REACHK) w += 1.5f * (float)((f >> 40) & 7);
+        if (f & F_REACHKR) w += 1.5f * (float)((f >> 43) & 7);
+        if (c & RARE_SLOTS) w += 4.0f;
+        pr.cost[p] = w;
     }
-    // Longest-first launch order of the (rule, scenario block) items: a rule's
-    // relative cost per candidate from the scans its flags select (the
-    // per-neighbour weights were measured on MI355X), so the expensive rules
-    // do not form the tail of the launch.
-    const int nblk = (T + 63) / 64;
-    std::vector<int32_t> order;
-    if (P > 0 && nblk > 0) {
-        std::vector<float> cost(P);
-        for (int p = 0; p < P; ++p) {
-            const uint64_t f = flg[p];
-            float c = 1.0f + 0.05f * lens[p];
-            if (f & (F_NS | F_MAXN | F_REACH2)) c += 1.0f;
-            if (f & F_REACH2) c += 1.0f;
-            if (f & (F_NSROB | F_R2ROB | F_REACHKR)) c += 1.0f;
-            if (f & F_R2ROB) c += 1.0f;
-            if (f & F_REGRET) c += 0.5f;
-            if (f & F_STRUCT) c += 2.0f;
-            if (f & F_NSTW) c += 1.0f;
-            if (f & F_ATRISK) c += 8.0f;
-            if (f & F_REACHK) c += 1.5f * (float)((f >> 40) & 7);
-            if (f & F_REACHKR) c += 1.5f * (float)((f >> 43) & 7);
-            if (comp[p] & RARE_SLOTS) c += 4.0f;
-            cost[p] = c;
-        }
-        order.resize((size_t)P);
-        for (int k = 0; k < P; ++k) order[k] = k;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
-    }
+    return SIGP_OK;
+}
+
+// A batch over `insts` (one instance, or several evaluating the same rules):
+// the rule arrays, one InstSlot per instance and, per size class, the
+// longest-first order of its (instance, rule) pairs, cost = rule cost x n^2.
+static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool multi,
+                              const PreparedRules &pr, int T, int group_size, sigp_batch **out) {
+    const int P = pr.P, K = (int)insts.size();
+    if ((long long)K * P > 0x7fffffffLL) return fail(SIGP_INVALID_ARG, "instances x rules exceeds 2^31");
     sigp_batch *bt = new sigp_batch();
-    bt->inst = inst;
+    bt->insts = insts;
+    bt->multi = multi;
     bt->P = P;
     bt->T = T;
     bt->group_size = group_size;
-    bt->waves_per_item = inst->waves_per_item;
-    bt->mapping = inst->mapping;
-    bt->rows_needed = P > 0 ? (int64_t)((P - 1) / group_size + 1) * T : 0;
+    bt->waves_per_item = insts[0]->waves_per_item;
+    bt->mapping = insts[0]->mapping;
+    bt->rows_needed = P == 0 ? 0 : (multi ? (int64_t)K * T : (int64_t)((P - 1) / group_size + 1) * T);
+    bt->n_out = (long long)K * P * T;
+    bt->has_rmod = pr.any_rmod;
+    bt->has_rare = (pr.any_rd & RARE_SLOTS) != 0;
+
+    std::vector<InstSlot> slots(K);
+    long long samp = 0;
+    for (int k = 0; k < K; ++k) {
+        slots[k].dev = insts[k]->dev;
+        slots[k].tabs = insts[k]->tabs;
+        slots[k].samp_off = samp;
+        slots[k].out_off = (long long)k * P * T;
+        if (multi) samp += (long long)T * insts[k]->n;
+    }
+    const int classes[3] = {4, 9, 16};
+    std::vector<int32_t> orders[3], samples[3];
+    // the autotuner times each mapping on about 2^19 simulations of a class
+    const long long tune_cap = std::max(1LL, (1LL << 19) / std::max(T, 1));
+    for (int c = 0; c < 3; ++c) {
+        LaunchUnit &u = bt->units[bt->n_units];
+        std::vector<std::pair<float, int32_t>> items;
+        for (int k = 0; k < K; ++k) {
+            if (nw_bucket(insts[k]->n) != classes[c]) continue;
+            u.max_n = std::max(u.max_n, insts[k]->n);
+            const float nn = (float)insts[k]->n * (float)insts[k]->n;
+            for (int p = 0; p < P; ++p) items.push_back({pr.cost[p] * nn, (int32_t)(k * P + p)});
+        }
+        if (items.empty()) continue;
+        std::stable_sort(items.begin(), items.end(),
+                         [](const std::pair<float, int32_t> &a, const std::pair<float, int32_t> &b) {
+                             return a.first > b.first;
+                         });
+        for (const auto &it : items) orders[bt->n_units].push_back(it.second);
+        const long long stride = ((long long)items.size() + tune_cap - 1) / tune_cap;
+        for (size_t i = 0; i < items.size(); i += (size_t)stride) samples[bt->n_units].push_back(items[i].second);
+        u.nwb = classes[c];
+        u.n_pairs = (long long)items.size();
+        u.tune_pairs = (long long)samples[bt->n_units].size();
+        ++bt->n_units;
+    }
+
+    // heaviest class first: it starts first and the others fill in beside it
+    {
+        int idx[3] = {0, 1, 2};
+        auto work = [&](int u) {
+            return (double)bt->units[u].n_pairs * bt->units[u].max_n * bt->units[u].max_n;
+        };
+        std::stable_sort(idx, idx + bt->n_units, [&](int a, int b) { return work(a) > work(b); });
+        LaunchUnit us[3];
+        std::vector<int32_t> os[3], ss[3];
+        for (int u = 0; u < bt->n_units; ++u) {
+            us[u] = bt->units[idx[u]];
+            os[u].swap(orders[idx[u]]);
+            ss[u].swap(samples[idx[u]]);
+        }
+        for (int u = 0; u < bt->n_units; ++u) {
+            bt->units[u] = us[u];
+            orders[u].swap(os[u]);
+            samples[u].swap(ss[u]);
+        }
+    }
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes + 8); return o; };
-    const size_t o_ops = take(ops.size() * 4), o_args = take(args.size() * 4), o_off = take(P * 4),
-                 o_len = take(P * 4), o_comp = take(P * 8), o_fl = take(P * 8),
-                 o_ord = take(order.size() * 4);
+    const size_t o_ops = take(pr.ops.size() * 4), o_args = take(pr.args.size() * 4),
+                 o_off = take((size_t)P * 4), o_len = take((size_t)P * 4), o_rd = take((size_t)P * 8),
+                 o_fl = take((size_t)P * 8), o_slots = take((size_t)K * sizeof(InstSlot));
+    size_t o_ord[3], o_smp[3];
+    for (int u = 0; u < bt->n_units; ++u) {
+        o_ord[u] = take(orders[u].size() * 4);
+        o_smp[u] = take(samples[u].size() * 4);
+    }
     char *base = nullptr;
-    hipError_t e = hipMalloc((void **)&base, off);
-    if (e != hipSuccess) { delete bt; return fail(SIGP_OOM, "batch allocation failed"); }
+    if (hipMalloc((void **)&base, off) != hipSuccess) {
+        delete bt;
+        return fail(SIGP_OOM, "batch allocation failed");
+    }
     bt->dbuf = base;
-    auto up = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
-        return bytes ? hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    auto up = [&](size_t o, const void *src, size_t bytes) -> bool {
+        return !bytes || hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (up(o_ops, ops.data(), ops.size() * 4) != hipSuccess ||
-        up(o_args, args.data(), args.size() * 4) != hipSuccess ||
-        up(o_off, offs.data(), P * 4) != hipSuccess || up(o_len, lens.data(), P * 4) != hipSuccess ||
-        up(o_comp, rdv.data(), P * 8) != hipSuccess || up(o_fl, flg.data(), P * 8) != hipSuccess ||
-        up(o_ord, order.data(), order.size() * 4) != hipSuccess) {
+    bool ok = up(o_ops, pr.ops.data(), pr.ops.size() * 4) && up(o_args, pr.args.data(), pr.args.size() * 4) &&
+              up(o_off, pr.offs.data(), (size_t)P * 4) && up(o_len, pr.lens.data(), (size_t)P * 4) &&
+              up(o_rd, pr.rd.data(), (size_t)P * 8) && up(o_fl, pr.flags.data(), (size_t)P * 8) &&
+              up(o_slots, slots.data(), (size_t)K * sizeof(InstSlot));
+    for (int u = 0; ok && u < bt->n_units; ++u)
+        ok = up(o_ord[u], orders[u].data(), orders[u].size() * 4) &&
+             up(o_smp[u], samples[u].data(), samples[u].size() * 4);
+    if (!ok) {
         (void)hipFree(base);
         delete bt;
         return fail(SIGP_DEVICE_ERROR, "batch upload failed");
@@ -1085,20 +1195,58 @@ extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *
     bt->rs.args = (cip)(base + o_args);
     bt->rs.off = (cip)(base + o_off);
     bt->rs.len = (cip)(base + o_len);
-    bt->rs.rd = (cup)(base + o_comp);
+    bt->rs.rd = (cup)(base + o_rd);
     bt->rs.flags = (cup)(base + o_fl);
-    bt->rs.order = order.empty() ? nullptr : (cip)(base + o_ord);
+    bt->rs.order = nullptr;
     bt->rs.P = P;
-    bt->rs.KL = max_k;
-    bt->has_rmod = any_rmod;
-    bt->has_rare = (any_rd & RARE_SLOTS) != 0;
-    bt->rs.SL = std::max(1, max_depth - 1);
+    bt->rs.KL = pr.max_k;
+    bt->rs.SL = std::max(1, pr.max_depth - 1);
+    bt->slots = (cslot)(base + o_slots);
+    for (int u = 0; u < bt->n_units; ++u) {
+        bt->units[u].order = (cip)(base + o_ord[u]);
+        bt->units[u].tune_order = (cip)(base + o_smp[u]);
+    }
     *out = bt;
     return SIGP_OK;
 }
 
+extern "C" sigp_status sigp_batch_create(sigp_instance *inst, const sigp_rules *rules, int32_t T,
+                                         int32_t group_size, const int64_t *masks, int32_t n_masks,
+                                         int32_t mask_group, sigp_batch **out) {
+    if (!out) return fail(SIGP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!inst || !rules) return fail(SIGP_INVALID_ARG, "inst and rules must be non-NULL");
+    if (T < 0) return fail(SIGP_INVALID_ARG, "T < 0");
+    if (group_size < 1) return fail(SIGP_INVALID_ARG, "group_size must be >= 1");
+    PreparedRules pr;
+    sigp_status st = prepare_rules(rules, masks, n_masks, mask_group, pr);
+    if (st != SIGP_OK) return st;
+    return make_batch({inst}, false, pr, T, group_size, out);
+}
+
+extern "C" sigp_status sigp_batch_create_multi(sigp_instance *const *insts, int32_t n_inst,
+                                               const sigp_rules *rules, int32_t T, const int64_t *masks,
+                                               int32_t n_masks, int32_t mask_group, sigp_batch **out) {
+    if (!out) return fail(SIGP_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    if (!insts || n_inst < 1 || !rules) return fail(SIGP_INVALID_ARG, "instances and rules must be given");
+    if (T < 0) return fail(SIGP_INVALID_ARG, "T < 0");
+    std::vector<sigp_instance *> v(insts, insts + n_inst);
+    for (sigp_instance *i : v)
+        if (!i) return fail(SIGP_INVALID_ARG, "an instance is NULL");
+    PreparedRules pr;
+    sigp_status st = prepare_rules(rules, masks, n_masks, mask_group, pr);
+    if (st != SIGP_OK) return st;
+    return make_batch(v, true, pr, T, std::max(pr.P, 1), out);
+}
+
 extern "C" sigp_status sigp_batch_destroy(sigp_batch *bt) {
     if (!bt) return SIGP_OK;
+    for (int i = 0; i < 2; ++i) {
+        if (bt->side[i]) (void)hipStreamDestroy(bt->side[i]);
+        if (bt->join[i]) (void)hipEventDestroy(bt->join[i]);
+    }
+    if (bt->fork) (void)hipEventDestroy(bt->fork);
     if (bt->dbuf) (void)hipFree(bt->dbuf);
     if (bt->pairs_tmp) (void)hipFree(bt->pairs_tmp);
     delete bt;
@@ -1121,6 +1269,10 @@ extern "C" sigp_status sigp_batch_configure(sigp_batch *bt, const sigp_launch_co
     if (st != SIGP_OK) return st;
     bt->waves_per_item = cfg->waves_per_item;
     bt->mapping = cfg->mapping;
+    for (int u = 0; u < bt->n_units; ++u) {   // drop per-class tuning
+        bt->units[u].mapping = SIGP_MAP_AUTO;
+        bt->units[u].waves = 0;
+    }
     return SIGP_OK;
 }
 
@@ -1136,7 +1288,7 @@ extern "C" sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_l
 extern "C" sigp_status sigp_batch_shape(const sigp_batch *bt, int32_t *n_rules, int64_t *n_pairs) {
     if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
     if (n_rules) *n_rules = bt->P;
-    if (n_pairs) *n_pairs = (int64_t)bt->P * bt->T;
+    if (n_pairs) *n_pairs = bt->n_out;
     return SIGP_OK;
 }
 
@@ -1151,22 +1303,24 @@ static int fit_waves(const sigp_batch *bt, int w) {
     return w;
 }
 
-// AUTO: candidate lanes below 16 scenarios per rule (scenario lanes would be
-// mostly idle), scenario lanes otherwise.
-static int resolved_mapping(const sigp_batch *bt) {
+// A size class's tuned mapping, else the batch's; AUTO: candidate lanes below
+// 16 scenarios per rule (scenario lanes would be mostly idle), scenario lanes
+// otherwise.
+static int resolved_mapping(const sigp_batch *bt, const LaunchUnit &u) {
+    if (u.mapping != SIGP_MAP_AUTO) return u.mapping;
     if (bt->mapping != SIGP_MAP_AUTO) return bt->mapping;
     return bt->T < 16 ? SIGP_MAP_CANDIDATES : SIGP_MAP_SCENARIOS;
 }
 
-// Wavefronts per workgroup: the configured count, else 4 for scenario lanes
-// and enough 64-candidate groups to cover n (at most 4) for candidate lanes;
-// then reduced until the workgroup's LDS fits.
-static int resolved_waves(const sigp_batch *bt, int mapping) {
-    int w = bt->waves_per_item;
+// Wavefronts per workgroup: the tuned or configured count, else 4 for
+// scenario lanes and enough 64-candidate groups to cover the class's largest
+// instance (at most 4) for candidate lanes; then reduced until LDS fits.
+static int resolved_waves(const sigp_batch *bt, const LaunchUnit &u, int mapping) {
+    int w = u.waves ? u.waves : bt->waves_per_item;
     if (w == 0) {
         w = 4;
         if (mapping == SIGP_MAP_CANDIDATES) {
-            const int need = (bt->inst->n + 63) / 64;
+            const int need = (u.max_n + 63) / 64;
             w = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
         }
     }
@@ -1174,35 +1328,45 @@ static int resolved_waves(const sigp_batch *bt, int mapping) {
 }
 
 template <int NW, bool RMOD, bool RARE>
-static void launch_variant(const sigp_batch *bt, int mapping, int w, const ScenSpec &S, double *out,
-                           hipStream_t stream) {
-    const sigp_instance *I = bt->inst;
+static void launch_variant(const sigp_batch *bt, const LaunchUnit &u, int mapping, int w,
+                           unsigned items, const ScenSpec &S, double *out, hipStream_t stream) {
+    RuleSet R = bt->rs;
+    R.order = u.order;
     const unsigned lds = (unsigned)(w * lds_per_wave(bt));
-    if (mapping == SIGP_MAP_CANDIDATES) {
-        const unsigned items = (unsigned)((long long)bt->P * S.T);
-        sim_cand_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
-    } else {
-        const unsigned items = (unsigned)((long long)bt->P * S.nblk);
-        sim_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(I->dev, I->tabs, bt->rs, S, out);
-    }
+    if (mapping == SIGP_MAP_CANDIDATES)
+        sim_cand_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
+    else
+        sim_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
 }
 
 template <int NW>
-static void launch_sim(const sigp_batch *bt, const double *sampled, double *out, hipStream_t stream) {
+static void launch_class(const sigp_batch *bt, const LaunchUnit &u, int mapping, int w, unsigned items,
+                         const ScenSpec &S, double *out, hipStream_t stream) {
+    if (bt->has_rmod) {
+        if (bt->has_rare) launch_variant<NW, true, true>(bt, u, mapping, w, items, S, out, stream);
+        else launch_variant<NW, true, false>(bt, u, mapping, w, items, S, out, stream);
+    } else {
+        if (bt->has_rare) launch_variant<NW, false, true>(bt, u, mapping, w, items, S, out, stream);
+        else launch_variant<NW, false, false>(bt, u, mapping, w, items, S, out, stream);
+    }
+}
+
+static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const double *sampled,
+                               double *out, hipStream_t stream) {
     ScenSpec S;
     S.sampled = sampled;
     S.T = bt->T;
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
-    const int mapping = resolved_mapping(bt);
-    const int w = resolved_waves(bt, mapping);
-    if (bt->has_rmod) {
-        if (bt->has_rare) launch_variant<NW, true, true>(bt, mapping, w, S, out, stream);
-        else launch_variant<NW, true, false>(bt, mapping, w, S, out, stream);
-    } else {
-        if (bt->has_rare) launch_variant<NW, false, true>(bt, mapping, w, S, out, stream);
-        else launch_variant<NW, false, false>(bt, mapping, w, S, out, stream);
-    }
+    const int mapping = resolved_mapping(bt, u);
+    const int w = resolved_waves(bt, u, mapping);
+    const long long items = u.n_pairs * (mapping == SIGP_MAP_CANDIDATES ? (long long)S.T : (long long)S.nblk);
+    if (items > 0xffffffffLL) return fail(SIGP_INVALID_ARG, "launch exceeds 2^32 workgroups");
+    if (u.nwb == 4) launch_class<4>(bt, u, mapping, w, (unsigned)items, S, out, stream);
+    else if (u.nwb == 9) launch_class<9>(bt, u, mapping, w, (unsigned)items, S, out, stream);
+    else launch_class<16>(bt, u, mapping, w, (unsigned)items, S, out, stream);
+    HIP_TRY(hipGetLastError());
+    return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,
@@ -1221,68 +1385,108 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
     double *pairs = out_pairs_dev;
     if (!pairs) {
         if (!out_mean_dev) return SIGP_OK;
-        if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->P * bt->T * 8));
+        if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->n_out * 8));
         pairs = bt->pairs_tmp;
     }
-    const int nwb = nw_bucket(bt->inst->n);
-    if (nwb == 4) launch_sim<4>(bt, sampled_dev, pairs, s);
-    else if (nwb == 9) launch_sim<9>(bt, sampled_dev, pairs, s);
-    else launch_sim<16>(bt, sampled_dev, pairs, s);
-    HIP_TRY(hipGetLastError());
+    if (bt->n_units == 1) {
+        const sigp_status st = launch_unit(bt, bt->units[0], sampled_dev, pairs, s);
+        if (st != SIGP_OK) return st;
+    } else {
+        // Size classes run concurrently: the first on the caller's stream, the
+        // others on the batch's own streams, forked from and joined back into
+        // the caller's stream with events (default-priority streams).
+        for (int i = 0; i + 1 < bt->n_units; ++i) {
+            if (!bt->side[i]) HIP_TRY(hipStreamCreateWithFlags(&bt->side[i], hipStreamNonBlocking));
+            if (!bt->join[i]) HIP_TRY(hipEventCreateWithFlags(&bt->join[i], hipEventDisableTiming));
+        }
+        if (!bt->fork) HIP_TRY(hipEventCreateWithFlags(&bt->fork, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(bt->fork, s));
+        for (int u = 0; u < bt->n_units; ++u) {
+            hipStream_t su = u == 0 ? s : bt->side[u - 1];
+            if (u > 0) HIP_TRY(hipStreamWaitEvent(su, bt->fork, 0));
+            const sigp_status st = launch_unit(bt, bt->units[u], sampled_dev, pairs, su);
+            if (st != SIGP_OK) return st;
+            if (u > 0) HIP_TRY(hipEventRecord(bt->join[u - 1], su));
+        }
+        for (int u = 1; u < bt->n_units; ++u) HIP_TRY(hipStreamWaitEvent(s, bt->join[u - 1], 0));
+    }
     if (out_mean_dev) {
-        mean_kernel<<<(bt->P + 255) / 256, 256, 0, s>>>(pairs, bt->P, bt->T, out_mean_dev);
+        const int rows = (int)(bt->n_out / bt->T);   // instances x rules
+        mean_kernel<<<(rows + 255) / 256, 256, 0, s>>>(pairs, rows, bt->T, out_mean_dev);
         HIP_TRY(hipGetLastError());
     }
+    return SIGP_OK;
+}
+
+// Times every mapping of one size class on its sample of pairs (twice each:
+// the first run of a kernel loads its code object) and keeps the fastest.
+static sigp_status tune_unit(sigp_batch *bt, LaunchUnit &u, const double *sampled, hipStream_t s,
+                             hipEvent_t e0, hipEvent_t e1) {
+    LaunchUnit probe = u;
+    probe.order = u.tune_order;
+    probe.n_pairs = u.tune_pairs;
+    struct Cand { int mapping, w; };
+    const Cand cands[6] = {{SIGP_MAP_SCENARIOS, 1}, {SIGP_MAP_SCENARIOS, 2}, {SIGP_MAP_SCENARIOS, 4},
+                           {SIGP_MAP_CANDIDATES, 1}, {SIGP_MAP_CANDIDATES, 2}, {SIGP_MAP_CANDIDATES, 4}};
+    const int groups = (u.max_n + 63) / 64;
+    int best_w = 0, best_m = SIGP_MAP_AUTO;
+    float best_ms = 0.0f;
+    for (const Cand &c : cands) {
+        if (fit_waves(bt, c.w) != c.w) continue;                           // LDS does not fit
+        if (c.mapping == SIGP_MAP_CANDIDATES && c.w > 1 && 32 * c.w >= groups * 64) continue;
+        probe.mapping = c.mapping;
+        probe.waves = c.w;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HIP_TRY(hipEventRecord(e0, s));
+            const sigp_status st = launch_unit(bt, probe, sampled, bt->pairs_tmp, s);
+            if (st != SIGP_OK) return st;
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        }
+        if (best_w == 0 || ms < best_ms) { best_ms = ms; best_w = c.w; best_m = c.mapping; }
+    }
+    u.mapping = best_m;
+    u.waves = best_w;
     return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_batch_autotune(sigp_batch *bt, const double *sampled_dev, int64_t n_rows,
                                            void *stream, sigp_launch_config *cfg_out) {
     if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
-    if (bt->P == 0 || bt->T == 0) {
+    if (bt->P == 0 || bt->T == 0 || bt->n_units == 0) {
         if (cfg_out) { cfg_out->waves_per_item = bt->waves_per_item; cfg_out->mapping = bt->mapping; }
         return SIGP_OK;
     }
-    if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->P * bt->T * 8));
+    if (!sampled_dev) return fail(SIGP_INVALID_ARG, "sampled_dev is NULL");
+    if (n_rows < bt->rows_needed) return fail(SIGP_INVALID_ARG, "sampled has too few rows");
+    if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->n_out * 8));
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    const int ncand_groups = (bt->inst->n + 63) / 64;
-    struct Cand { int mapping, w; };
-    const Cand cands[6] = {{SIGP_MAP_SCENARIOS, 1}, {SIGP_MAP_SCENARIOS, 2}, {SIGP_MAP_SCENARIOS, 4},
-                           {SIGP_MAP_CANDIDATES, 1}, {SIGP_MAP_CANDIDATES, 2}, {SIGP_MAP_CANDIDATES, 4}};
-    const int keep_w = bt->waves_per_item, keep_m = bt->mapping;
-    int best_w = 0, best_m = SIGP_MAP_AUTO;
-    float best_ms = 0.0f;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(SIGP_DEVICE_ERROR, "hipEventCreate failed");
+    }
     sigp_status st = SIGP_OK;
-    for (const Cand &c : cands) {
-        if (fit_waves(bt, c.w) != c.w) continue;                         // LDS does not fit
-        if (c.mapping == SIGP_MAP_CANDIDATES && c.w > 1 && 32 * c.w >= ncand_groups * 64) continue;
-        bt->waves_per_item = c.w;
-        bt->mapping = c.mapping;
-        float ms = 0.0f;
-        for (int rep = 0; rep < 2 && st == SIGP_OK; ++rep) {   // the first run loads the code object
-            (void)hipEventRecord(e0, s);
-            st = sigp_batch_run(bt, sampled_dev, n_rows, bt->pairs_tmp, nullptr, stream);
-            (void)hipEventRecord(e1, s);
-            if (st != SIGP_OK) break;
-            if (hipEventSynchronize(e1) != hipSuccess) { st = fail(SIGP_DEVICE_ERROR, "autotune run failed"); break; }
-            (void)hipEventElapsedTime(&ms, e0, e1);
-        }
-        if (st != SIGP_OK) break;
-        if (best_w == 0 || ms < best_ms) { best_ms = ms; best_w = c.w; best_m = c.mapping; }
+    int heaviest = 0;
+    double heavy_w = -1.0;
+    for (int u = 0; u < bt->n_units && st == SIGP_OK; ++u) {
+        st = tune_unit(bt, bt->units[u], sampled_dev, s, e0, e1);
+        const double w = (double)bt->units[u].n_pairs * bt->units[u].max_n * bt->units[u].max_n;
+        if (w > heavy_w) { heavy_w = w; heaviest = u; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (st != SIGP_OK || best_w == 0) {
-        bt->waves_per_item = keep_w;
-        bt->mapping = keep_m;
+    if (st != SIGP_OK) {
+        for (int u = 0; u < bt->n_units; ++u) { bt->units[u].mapping = SIGP_MAP_AUTO; bt->units[u].waves = 0; }
         return st;
     }
-    bt->waves_per_item = best_w;
-    bt->mapping = best_m;
-    if (cfg_out) { cfg_out->waves_per_item = best_w; cfg_out->mapping = best_m; }
+    if (cfg_out) {
+        cfg_out->waves_per_item = bt->units[heaviest].waves;
+        cfg_out->mapping = bt->units[heaviest].mapping;
+    }
     return SIGP_OK;
 }
 
@@ -1296,6 +1500,30 @@ static sigp_status ensure(double **buf, size_t *cap, size_t bytes) {
     return SIGP_OK;
 }
 
+// Uploads `sbytes` of scenarios, runs the batch, downloads results; the
+// buffers are the first instance's cached staging buffers.
+static sigp_status run_host(sigp_batch *bt, const double *sampled, size_t sbytes, int64_t n_rows,
+                            double *out_pairs, double *out_mean) {
+    sigp_instance *inst = bt->insts[0];
+    const size_t obytes = (size_t)bt->n_out * 8, mbytes = (size_t)(bt->T ? bt->n_out / bt->T : 0) * 8;
+    sigp_status st;
+    if ((st = ensure(&inst->scen_buf, &inst->scen_cap, sbytes)) != SIGP_OK ||
+        (st = ensure(&inst->out_buf, &inst->out_cap, obytes)) != SIGP_OK ||
+        (st = ensure(&inst->mean_buf, &inst->mean_cap, mbytes)) != SIGP_OK)
+        return st;
+    hipError_t e = sbytes ? hipMemcpy(inst->scen_buf, sampled, sbytes, hipMemcpyHostToDevice) : hipSuccess;
+    if (e != hipSuccess) return fail(SIGP_DEVICE_ERROR, "scenario upload failed");
+    st = sigp_batch_run(bt, inst->scen_buf, n_rows, inst->out_buf, out_mean ? inst->mean_buf : nullptr, nullptr);
+    if (st != SIGP_OK) return st;
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(SIGP_DEVICE_ERROR, std::string("kernel failed: ") + hipGetErrorString(e));
+    if (out_pairs && obytes && hipMemcpy(out_pairs, inst->out_buf, obytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SIGP_DEVICE_ERROR, "result download failed");
+    if (out_mean && mbytes && hipMemcpy(out_mean, inst->mean_buf, mbytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(SIGP_DEVICE_ERROR, "mean download failed");
+    return SIGP_OK;
+}
+
 extern "C" sigp_status sigp_evaluate(sigp_instance *inst, const sigp_rules *rules, const double *sampled,
                                      int64_t n_rows, int32_t T, int32_t group_size, const int64_t *masks,
                                      int32_t n_masks, int32_t mask_group, double *out_pairs,
@@ -1304,10 +1532,9 @@ extern "C" sigp_status sigp_evaluate(sigp_instance *inst, const sigp_rules *rule
     sigp_batch *bt = nullptr;
     sigp_status st = sigp_batch_create(inst, rules, T, group_size, masks, n_masks, mask_group, &bt);
     if (st != SIGP_OK) return st;
-    const int P = bt->P;
-    if (P == 0 || T == 0) {
+    if (bt->P == 0 || T == 0) {
         sigp_batch_destroy(bt);
-        if (out_mean && P > 0) return fail(SIGP_INVALID_ARG, "mean over T = 0 scenarios");
+        if (out_mean && rules->n_rules > 0) return fail(SIGP_INVALID_ARG, "mean over T = 0 scenarios");
         return SIGP_OK;
     }
     if (!sampled) { sigp_batch_destroy(bt); return fail(SIGP_INVALID_ARG, "sampled is NULL"); }
@@ -1315,28 +1542,27 @@ extern "C" sigp_status sigp_evaluate(sigp_instance *inst, const sigp_rules *rule
         sigp_batch_destroy(bt);
         return fail(SIGP_INVALID_ARG, "sampled has too few rows");
     }
-    const size_t sbytes = (size_t)n_rows * inst->n * 8, obytes = (size_t)P * T * 8;
-    if ((st = ensure(&inst->scen_buf, &inst->scen_cap, sbytes)) != SIGP_OK ||
-        (st = ensure(&inst->out_buf, &inst->out_cap, obytes)) != SIGP_OK ||
-        (st = ensure(&inst->mean_buf, &inst->mean_cap, (size_t)P * 8)) != SIGP_OK) {
+    st = run_host(bt, sampled, (size_t)n_rows * inst->n * 8, n_rows, out_pairs, out_mean);
+    sigp_batch_destroy(bt);
+    return st;
+}
+
+extern "C" sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst,
+                                           const sigp_rules *rules, const double *sampled, int32_t T,
+                                           const int64_t *masks, int32_t n_masks, int32_t mask_group,
+                                           double *out_pairs, double *out_mean) {
+    sigp_batch *bt = nullptr;
+    sigp_status st = sigp_batch_create_multi(insts, n_inst, rules, T, masks, n_masks, mask_group, &bt);
+    if (st != SIGP_OK) return st;
+    if (bt->P == 0 || T == 0) {
         sigp_batch_destroy(bt);
-        return st;
+        if (out_mean && rules->n_rules > 0) return fail(SIGP_INVALID_ARG, "mean over T = 0 scenarios");
+        return SIGP_OK;
     }
-    hipError_t e = sbytes ? hipMemcpy(inst->scen_buf, sampled, sbytes, hipMemcpyHostToDevice) : hipSuccess;
-    if (e != hipSuccess) { sigp_batch_destroy(bt); return fail(SIGP_DEVICE_ERROR, "scenario upload failed"); }
-    st = sigp_batch_run(bt, inst->scen_buf, n_rows, inst->out_buf, out_mean ? inst->mean_buf : nullptr, nullptr);
-    if (st == SIGP_OK) {
-        e = hipDeviceSynchronize();
-        if (e != hipSuccess) st = fail(SIGP_DEVICE_ERROR, std::string("kernel failed: ") + hipGetErrorString(e));
-    }
-    if (st == SIGP_OK && out_pairs) {
-        e = hipMemcpy(out_pairs, inst->out_buf, obytes, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = fail(SIGP_DEVICE_ERROR, "result download failed");
-    }
-    if (st == SIGP_OK && out_mean) {
-        e = hipMemcpy(out_mean, inst->mean_buf, (size_t)P * 8, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) st = fail(SIGP_DEVICE_ERROR, "mean download failed");
-    }
+    if (!sampled) { sigp_batch_destroy(bt); return fail(SIGP_INVALID_ARG, "sampled is NULL"); }
+    size_t doubles = 0;
+    for (int k = 0; k < n_inst; ++k) doubles += (size_t)T * insts[k]->n;
+    st = run_host(bt, sampled, doubles * 8, (int64_t)n_inst * T, out_pairs, out_mean);
     sigp_batch_destroy(bt);
     return st;
 }

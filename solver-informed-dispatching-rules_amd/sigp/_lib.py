@@ -18,7 +18,8 @@ lib_path = os.path.join(_HERE, "libsigp.so")
 EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp_instance_destroy",
             "sigp_zstats", "sigp_evaluate", "sigp_batch_create", "sigp_batch_run",
             "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math", "sigp_batch_configure",
-            "sigp_batch_autotune", "sigp_instance_configure"]
+            "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
+            "sigp_evaluate_multi"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
            4: "SIGP_NO_DEVICE"}
@@ -84,6 +85,8 @@ def lib() -> C.CDLL:
         "sigp_batch_configure": ([vp, vp], C.c_int),
         "sigp_batch_autotune": ([vp, vp, i64, vp, vp], C.c_int),
         "sigp_instance_configure": ([vp, vp], C.c_int),
+        "sigp_batch_create_multi": ([vp, i32, vp, i32, vp, i32, i32, vp], C.c_int),
+        "sigp_evaluate_multi": ([vp, i32, vp, vp, i32, vp, i32, i32, vp, vp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(L, name)

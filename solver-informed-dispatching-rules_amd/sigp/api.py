@@ -14,8 +14,8 @@ from ._lib import MAP_AUTO, MAP_CANDIDATES, MAP_SCENARIOS, InstanceDesc, LaunchC
 from .instance import Instance
 from .rules import N_TERMS, Packed
 
-__all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info", "MAP_AUTO",
-           "MAP_SCENARIOS", "MAP_CANDIDATES"]
+__all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info", "evaluate_multi",
+           "batch_multi", "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -78,16 +78,24 @@ def device_math(which: str, x, y=None) -> np.ndarray:
 
 
 class Batch:
-    """Rules and masks resident on the device (sigp_batch_*)."""
+    """Rules and masks resident on the device (sigp_batch_*), over one
+    instance or, from batch_multi, several."""
 
     def __init__(self, ctx: "FactoryContext", pk: Packed, T: int, group_size: int, masks,
-                 mask_group: int):
+                 mask_group: int, _multi: Optional[Sequence["FactoryContext"]] = None):
         keep: list = []
         r = _rules(pk, keep)
         m = _masks(masks)
         self._h = C.c_void_p()
-        check(lib().sigp_batch_create(ctx._h, C.byref(r), T, group_size, _ptr(m), m.shape[0],
-                                      mask_group, C.byref(self._h)))
+        if _multi is None:
+            check(lib().sigp_batch_create(ctx._h, C.byref(r), T, group_size, _ptr(m), m.shape[0],
+                                          mask_group, C.byref(self._h)))
+            self.ctxs = [ctx]
+        else:
+            hs = (C.c_void_p * len(_multi))(*[c._h for c in _multi])
+            check(lib().sigp_batch_create_multi(hs, len(_multi), C.byref(r), T, _ptr(m),
+                                                m.shape[0], mask_group, C.byref(self._h)))
+            self.ctxs = list(_multi)
         self.ctx = ctx
         self.P = pk.n_rules
         self.T = T
@@ -193,3 +201,32 @@ class FactoryContext:
               mask_group: Optional[int] = None) -> Batch:
         P = max(pk.n_rules, 1)
         return Batch(self, pk, T, group_size or P, masks, mask_group or P)
+
+
+def batch_multi(ctxs: Sequence[FactoryContext], pk: Packed, T: int, masks,
+                mask_group: Optional[int] = None) -> Batch:
+    """Every rule of pk on every instance of ctxs (sigp_batch_create_multi);
+    run with the instances' scenario blocks back to back, n_rows = len(ctxs) * T,
+    results out[(k * P + p) * T + s]."""
+    P = max(pk.n_rules, 1)
+    return Batch(ctxs[0], pk, T, P, masks, mask_group or P, _multi=list(ctxs))
+
+
+def evaluate_multi(ctxs: Sequence[FactoryContext], pk: Packed, blocks: Sequence[np.ndarray], T: int,
+                   masks, mask_group: Optional[int] = None, want_mean: bool = False):
+    """sigp_evaluate_multi: blocks[k] holds instance k's T scenario rows;
+    returns out[(k * P + p) * T + s] (and the n_inst * P means)."""
+    keep: list = []
+    r = _rules(pk, keep)
+    m = _masks(masks)
+    K, P = len(ctxs), pk.n_rules
+    for k, (c, b) in enumerate(zip(ctxs, blocks)):
+        if np.asarray(b).size != T * c.inst.n:
+            raise ValueError("block %d must hold T x n = %d x %d durations" % (k, T, c.inst.n))
+    flat = _f64(np.concatenate([np.asarray(b, dtype=np.float64).ravel() for b in blocks]))
+    hs = (C.c_void_p * K)(*[c._h for c in ctxs])
+    out = np.zeros(K * P * T, dtype=np.float64)
+    mean = np.zeros(K * P, dtype=np.float64) if want_mean else None
+    check(lib().sigp_evaluate_multi(hs, K, C.byref(r), _ptr(flat), T, _ptr(m), m.shape[0],
+                                    mask_group or max(P, 1), _ptr(out), _ptr(mean)))
+    return (out, mean) if want_mean else out

@@ -17,8 +17,8 @@ from .scenarios import gen_durations
 
 TEST_SEED = 1_000_000   # engine/gp_engine.cu:22
 
-__all__ = ["Workload", "plain_small", "large", "gp_generation", "gp_reference_generation",
-           "random_rules", "TEST_SEED"]
+__all__ = ["Workload", "MultiWorkload", "plain_small", "large", "gp_generation",
+           "gp_reference_generation", "batched_uneven", "random_rules", "TEST_SEED"]
 
 
 @dataclass
@@ -86,6 +86,39 @@ def gp_reference_generation(seeds: int = 30, pop: int = 1024) -> Workload:
                               for s in range(seeds)])
     return Workload("gp-reference-generation n=100 m=3 %d seeds x %d rules x 1" % (seeds, pop),
                     inst, trees, pack_trees(trees), ext29_mask(), sampled, 1, pop)
+
+
+@dataclass
+class MultiWorkload:
+    name: str
+    insts: List[Instance]
+    trees: List[Node]
+    packed: Packed
+    mask: np.ndarray
+    blocks: List[np.ndarray]   # per instance: T x n_k scenario rows
+    n_scen: int
+
+    @property
+    def n_sims(self) -> int:
+        return len(self.insts) * self.packed.n_rules * self.n_scen
+
+
+def batched_uneven(n_inst: int = 40, n_rules: int = 256, n_scen: int = 500, seed: int = 1) -> MultiWorkload:
+    """Many uneven instances in one batch: n drawn from {48, 96, ..., 288},
+    uniform and clustered coordinates alternating, m in {1..4}, windows
+    U[10,200]; ramped half-and-half rules of 3-127 nodes over EXTRA-18; each
+    instance with its own Gamma(4, mean 10) scenarios (base 1,000,000)."""
+    rng = random.Random(seed)
+    insts = []
+    for k in range(n_inst):
+        n = rng.choice([48, 96, 144, 192, 240, 288])
+        m = rng.randint(1, 4)
+        gen = synthetic_uniform if k % 2 == 0 else synthetic_clustered
+        insts.append(gen(n=n, m=m, seed=100 + k, width=(10.0, 200.0)))
+    trees = random_rules(n_rules, EXTRA, seed=0, min_depth=1, max_depth=6)
+    blocks = [gen_durations(inst.durations, n_scen, TEST_SEED, mode="gamma", shape=4.0) for inst in insts]
+    return MultiWorkload("batched-uneven %d instances x %d rules x %d" % (n_inst, n_rules, n_scen),
+                         insts, trees, pack_trees(trees), term_mask_from(EXTRA), blocks, n_scen)
 
 
 def ext29_mask() -> np.ndarray:
