@@ -98,6 +98,9 @@ def describe(kind, wl):
     if kind == "gp-generation":
         return ("one GP generation: clustered instance n=100 m=3, %d rules (ramped half-and-half, "
                 "depth 3-9 levels, NS-11 + EXTRA-18) x %d Gamma(4) scenarios" % (P, T))
+    if kind == "large":
+        return ("large: clustered + uniform instance n=288 m=4 (windows U[20,200]), %d rules (up to 127 "
+                "nodes, EXTRA-18) x %d Gamma(2, mean 10) scenarios" % (P, T))
     if kind == "batched":
         ns = sorted(i.n for i in wl.insts)
         return ("batched uneven: %d instances (n %d-%d, m 1-4, uniform/clustered, windows U[10,200]) "
@@ -113,15 +116,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--rules", type=int, default=1024)
-    ap.add_argument("--scenarios", type=int, default=500)
-    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched"],
+    ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
+    ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
+    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
                          "gp-reference: one launch of the reference's GP loop (30 seeds x 1,024 "
                          "rules x 1 scenario); batched: 40 uneven instances (n 48-288) x 256 "
-                         "rules x 500 scenarios in one multi-instance batch")
+                         "rules x 500 scenarios in one multi-instance batch; large: n=288 m=4, "
+                         "4,096 rules x 2,000 scenarios")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--waves", type=int, default=0,
@@ -138,15 +142,18 @@ def main():
 
     import torch
     from sigp.api import FactoryContext, batch_multi, device_info
-    from sigp.workloads import batched_uneven, gp_generation, gp_reference_generation, plain_small
+    from sigp.workloads import (batched_uneven, gp_generation, gp_reference_generation, large,
+                                plain_small)
 
     t0 = time.perf_counter()
     if args.workload == "plain":
-        wl = plain_small(n_rules=args.rules, n_scen=args.scenarios)
+        wl = plain_small(n_rules=args.rules or 1024, n_scen=args.scenarios or 500)
     elif args.workload == "gp-generation":
         wl = gp_generation()
     elif args.workload == "gp-reference":
         wl = gp_reference_generation()
+    elif args.workload == "large":
+        wl = large(n_rules=args.rules or 4096, n_scen=args.scenarios or 2000)
     else:
         wl = batched_uneven()
     log("workload built in %.1fs: %s" % (time.perf_counter() - t0, wl.name))
