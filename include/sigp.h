@@ -185,6 +185,63 @@ sigp_status sigp_batch_shape(const sigp_batch *batch, int32_t *n_rules, int64_t 
 sigp_status sigp_device_math(int32_t which, const double *x, const double *y, int64_t count,
                              double *out);
 
+/* ---- phenotype (no counterpart in the reference) --------------------------
+ * Decision situations: per situation the state of a route under
+ * construction (current POI, clock, vehicles left after the current one,
+ * visited customers) and n_cand customers to rank. */
+typedef struct {
+    int32_t n_sit, n_cand;        /* 1 <= n_sit * n_cand <= 1024 */
+    const int32_t *cur;           /* n_sit: current POI, 0 = depot */
+    const double *ct;             /* n_sit: current time */
+    const int32_t *remaining;     /* n_sit: vehicles after the current one */
+    const uint8_t *visited;       /* n_sit x n: 1 = customer visited */
+    const int32_t *cand;          /* n_sit x n_cand: customers to rank (feasible ones) */
+} sigp_situations;
+
+/* A rule's phenotype: for rule p, situation s and candidate slot j,
+ * out_ranks[(p * n_sit + s) * n_cand + j] = the candidate's rank by the
+ * rule's priority (0 = the one pick_best would choose), computed on the
+ * device by the simulator's own per-candidate evaluation
+ * (engine/sim_core.hpp:73-203).  Priorities at or below -1e300 (and NaN)
+ * count as -1e300; equal priorities rank by candidate slot.  A candidate
+ * that is visited or infeasible in its situation counts as -1e300.
+ * Masks as in sigp_evaluate.  Synchronous. */
+sigp_status sigp_phenotype(sigp_instance *inst, const sigp_rules *rules, const int64_t *masks,
+                           int32_t n_masks, int32_t mask_group, const sigp_situations *sit,
+                           int32_t *out_ranks);
+
+/* ---- surrogate network (no counterpart in the reference) -----------------
+ * A 512 -> 256 -> 256 -> 1 ReLU MLP that predicts a rule's mean reward from
+ * its phenotype (sigp_phenotype), trained and run on the device: FP32
+ * parameters and Adam state, BF16 copies of the weight matrices and BF16
+ * activations on the matrix cores (v_mfma_f32_16x16x32_bf16), FP32
+ * accumulation, FP32 pre-activations, outputs and gradients.  Features are
+ * row-major float rows of SIGP_MLP_IN values; parameters are laid out as
+ * W1[512][256], b1[256], W2[256][256], b2[256], w3[256], b3 (SIGP_MLP_NPARAMS
+ * floats).  Loss: mean squared error over the batch. */
+#define SIGP_MLP_IN 512
+#define SIGP_MLP_HIDDEN 256
+#define SIGP_MLP_NPARAMS 197377
+typedef struct sigp_mlp sigp_mlp;
+
+/* Glorot-uniform weights and zero biases from a splitmix64 stream of seed. */
+sigp_status sigp_mlp_create(uint64_t seed, sigp_mlp **out);
+sigp_status sigp_mlp_destroy(sigp_mlp *net);
+sigp_status sigp_mlp_get_params(sigp_mlp *net, float *out);
+/* Replaces the parameters and resets the Adam state. */
+sigp_status sigp_mlp_set_params(sigp_mlp *net, const float *in);
+/* Loss gradient of one batch (1..1024 rows) with respect to every parameter,
+ * without updating; loss_out (nullable) receives the batch loss. */
+sigp_status sigp_mlp_gradients(sigp_mlp *net, const float *x, const float *t, int32_t rows,
+                               float *grad_out, float *loss_out);
+/* `epochs` passes of Adam (lr; beta 0.9 / 0.999; eps 1e-8) over `rows`
+ * examples in batches of `batch` (1..1024), shuffled per epoch from
+ * shuffle_seed; epoch_loss (nullable) receives each epoch's mean loss. */
+sigp_status sigp_mlp_train(sigp_mlp *net, const float *x, const float *t, int64_t rows,
+                           int32_t epochs, int32_t batch, float lr, uint64_t shuffle_seed,
+                           float *epoch_loss);
+sigp_status sigp_mlp_predict(sigp_mlp *net, const float *x, int64_t rows, float *y_out);
+
 #ifdef __cplusplus
 }
 #endif

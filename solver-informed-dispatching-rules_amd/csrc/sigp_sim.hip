@@ -55,6 +55,9 @@ static sigp_status fail(sigp_status st, const std::string &msg) {
 
 extern "C" const char *sigp_last_error(void) { return g_err.c_str(); }
 
+// Error reporting for the library's other translation units.
+sigp_status sigp_internal_fail(sigp_status st, const std::string &msg) { return fail(st, msg); }
+
 constexpr uint64_t B(int s) { return 1ull << s; }
 
 // ---------------------------------------------------------------------------
@@ -600,6 +603,67 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
         }
     }
     if (threadIdx.x == 0) out[slots[k].out_off + (size_t)p * S.T + s] = total;
+}
+
+// Decision situations of the phenotype: per situation the state of a route
+// under construction and the candidates to rank.
+struct PhenoSpec {
+    int n_sit, n_cand;
+    const int *cur, *rem, *cand;   // n_sit, n_sit, n_sit x n_cand
+    const double *ct;              // n_sit
+    const uint32_t *vis;           // n_sit x 16 words: visited customers
+};
+
+// A rule's phenotype: the rank of each candidate of each situation by the
+// rule's priority, 0 = the candidate pick_best would choose.  Priorities
+// come from the same per-candidate evaluation as the simulation
+// (engine/sim_core.hpp:73-203, via decide on one candidate), clamped below
+// at -1e300 (NaN counts as -1e300); equal priorities rank by candidate
+// position, as the strict ">" scan would.  One workgroup per rule; thread t
+// evaluates the (situation, candidate) pairs t, t + 64W, ...
+template <int NW, bool RMOD, bool RARE>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 256)))
+void pheno_kernel(cslot slots, RuleSet R, PhenoSpec Ph, int32_t *__restrict__ out) {
+    __shared__ double spri[1024];
+    extern __shared__ double lds_rows[];
+    const int p = (int)blockIdx.x;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
+    const DevInst I = slots[0].dev;
+    const FusedTabs F = slots[0].tabs;
+    const int off = R.off[p], L = R.len[p];
+    const cip ops = R.ops + off;
+    const cip args = R.args + 2 * (size_t)off;
+    const uint64_t rd = R.rd[p];
+    const uint64_t fl = R.flags[p];
+    double *tv = lds_rows + (size_t)wv * (R.KL + R.SL) * 64 + lane;
+    double *st = tv + (size_t)R.KL * 64;
+    const int pairs = Ph.n_sit * Ph.n_cand;
+    for (int q0 = 0; q0 < pairs; q0 += (int)blockDim.x) {
+        const int q = q0 + (int)threadIdx.x;
+        const bool live = q < pairs;
+        const int si = live ? q / Ph.n_cand : 0;
+        const int c = live ? Ph.cand[q] : 0;
+        uint32_t vis[NW];
+#pragma unroll
+        for (int w = 0; w < NW; ++w) vis[w] = Ph.vis[(size_t)si * 16 + w];
+        double bp = -1e300;
+        if (live)
+            (void)decide<NW, RMOD, RARE>(I, F, ops, args, L, rd, fl, Ph.cur[si], Ph.ct[si], Ph.rem[si],
+                                         vis, c, I.n, tv, st, bp);
+        if (live) spri[q] = bp;
+    }
+    __syncthreads();
+    for (int q = (int)threadIdx.x; q < pairs; q += (int)blockDim.x) {
+        const int base = (q / Ph.n_cand) * Ph.n_cand, j = q - base;
+        const double mine = spri[q];
+        int rank = 0;
+        for (int k = 0; k < Ph.n_cand; ++k) {
+            const double o = spri[base + k];
+            rank += (o > mine || (o == mine && k < j)) ? 1 : 0;
+        }
+        out[(size_t)p * pairs + q] = rank;
+    }
 }
 
 // Per-rule mean in scenario order, then / T (engine/gp_engine.cu:76-81).
@@ -1565,6 +1629,85 @@ extern "C" sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t 
     st = run_host(bt, sampled, doubles * 8, (int64_t)n_inst * T, out_pairs, out_mean);
     sigp_batch_destroy(bt);
     return st;
+}
+
+extern "C" sigp_status sigp_phenotype(sigp_instance *inst, const sigp_rules *rules, const int64_t *masks,
+                                      int32_t n_masks, int32_t mask_group, const sigp_situations *sit,
+                                      int32_t *out_ranks) {
+    if (!inst || !rules || !sit || !out_ranks) return fail(SIGP_INVALID_ARG, "NULL argument");
+    const int S = sit->n_sit, C = sit->n_cand, n = inst->n;
+    if (S < 1 || C < 1 || (long long)S * C > 1024)
+        return fail(SIGP_INVALID_ARG, "need 1 <= n_sit * n_cand <= 1024");
+    if (!sit->cur || !sit->ct || !sit->remaining || !sit->visited || !sit->cand)
+        return fail(SIGP_INVALID_ARG, "situation arrays must be non-NULL");
+    std::vector<uint32_t> words((size_t)S * 16, 0u);
+    for (int k = 0; k < S; ++k) {
+        if (sit->cur[k] < 0 || sit->cur[k] > n) return fail(SIGP_INVALID_ARG, "situation cur out of range");
+        if (sit->remaining[k] < 0) return fail(SIGP_INVALID_ARG, "situation remaining < 0");
+        for (int i = 0; i < n; ++i)
+            if (sit->visited[(size_t)k * n + i]) words[(size_t)k * 16 + (i >> 5)] |= 1u << (i & 31);
+        for (int j = 0; j < C; ++j) {
+            const int c = sit->cand[(size_t)k * C + j];
+            if (c < 0 || c >= n) return fail(SIGP_INVALID_ARG, "candidate out of range");
+        }
+    }
+    PreparedRules pr;
+    sigp_status st = prepare_rules(rules, masks, n_masks, mask_group, pr);
+    if (st != SIGP_OK) return st;
+    if (pr.P == 0) return SIGP_OK;
+    sigp_batch *bt = nullptr;
+    if ((st = make_batch({inst}, false, pr, 1, std::max(pr.P, 1), &bt)) != SIGP_OK) return st;
+    // situations + results in one allocation
+    const size_t o_cur = 0, o_rem = align256(S * 4 + 8), o_cand = o_rem + align256(S * 4 + 8),
+                 o_ct = o_cand + align256((size_t)S * C * 4 + 8), o_vis = o_ct + align256(S * 8 + 8),
+                 o_out = o_vis + align256(words.size() * 4 + 8),
+                 total = o_out + align256((size_t)pr.P * S * C * 4 + 8);
+    char *d = nullptr;
+    if (hipMalloc((void **)&d, total) != hipSuccess) {
+        sigp_batch_destroy(bt);
+        return fail(SIGP_OOM, "phenotype allocation failed");
+    }
+    hipError_t e = hipMemcpy(d + o_cur, sit->cur, S * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + o_rem, sit->remaining, S * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + o_cand, sit->cand, (size_t)S * C * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + o_ct, sit->ct, S * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d + o_vis, words.data(), words.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        PhenoSpec Ph;
+        Ph.n_sit = S;
+        Ph.n_cand = C;
+        Ph.cur = (const int *)(d + o_cur);
+        Ph.rem = (const int *)(d + o_rem);
+        Ph.cand = (const int *)(d + o_cand);
+        Ph.ct = (const double *)(d + o_ct);
+        Ph.vis = (const uint32_t *)(d + o_vis);
+        int w = 4;   // the kernel's static LDS holds 1024 priorities
+        while (w > 1 && 1024 * sizeof(double) + (size_t)w * lds_per_wave(bt) > LDS_LIMIT) w >>= 1;
+        const unsigned lds = (unsigned)(w * lds_per_wave(bt));
+        int32_t *outd = (int32_t *)(d + o_out);
+        const int nwb = nw_bucket(n);
+#define PHENO(NWV)                                                                                   \
+    do {                                                                                             \
+        if (bt->has_rmod) {                                                                          \
+            if (bt->has_rare) pheno_kernel<NWV, true, true><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
+            else pheno_kernel<NWV, true, false><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
+        } else {                                                                                     \
+            if (bt->has_rare) pheno_kernel<NWV, false, true><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
+            else pheno_kernel<NWV, false, false><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
+        }                                                                                            \
+    } while (0)
+        if (nwb == 4) PHENO(4);
+        else if (nwb == 9) PHENO(9);
+        else PHENO(16);
+#undef PHENO
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(out_ranks, outd, (size_t)pr.P * S * C * 4, hipMemcpyDeviceToHost);
+    }
+    (void)hipFree(d);
+    sigp_batch_destroy(bt);
+    if (e != hipSuccess) return fail(SIGP_DEVICE_ERROR, std::string("phenotype failed: ") + hipGetErrorString(e));
+    return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_device_math(int32_t which, const double *x, const double *y, int64_t count,

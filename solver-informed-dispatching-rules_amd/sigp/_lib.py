@@ -8,7 +8,8 @@ import ctypes as C
 import os
 from typing import Optional
 
-__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "EXPORTED",
+__all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "Situations",
+           "EXPORTED",
            "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -19,7 +20,9 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_zstats", "sigp_evaluate", "sigp_batch_create", "sigp_batch_run",
             "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math", "sigp_batch_configure",
             "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
-            "sigp_evaluate_multi"]
+            "sigp_evaluate_multi", "sigp_phenotype", "sigp_mlp_create", "sigp_mlp_destroy",
+            "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
+            "sigp_mlp_predict"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
            4: "SIGP_NO_DEVICE"}
@@ -43,6 +46,11 @@ MAP_AUTO, MAP_SCENARIOS, MAP_CANDIDATES = 0, 1, 2
 
 class LaunchConfig(C.Structure):
     _fields_ = [("waves_per_item", C.c_int32), ("mapping", C.c_int32)]
+
+
+class Situations(C.Structure):
+    _fields_ = [("n_sit", C.c_int32), ("n_cand", C.c_int32), ("cur", C.c_void_p), ("ct", C.c_void_p),
+                ("remaining", C.c_void_p), ("visited", C.c_void_p), ("cand", C.c_void_p)]
 
 
 class Rules(C.Structure):
@@ -87,6 +95,14 @@ def lib() -> C.CDLL:
         "sigp_instance_configure": ([vp, vp], C.c_int),
         "sigp_batch_create_multi": ([vp, i32, vp, i32, vp, i32, i32, vp], C.c_int),
         "sigp_evaluate_multi": ([vp, i32, vp, vp, i32, vp, i32, i32, vp, vp], C.c_int),
+        "sigp_phenotype": ([vp, vp, vp, i32, i32, vp, vp], C.c_int),
+        "sigp_mlp_create": ([C.c_uint64, vp], C.c_int),
+        "sigp_mlp_destroy": ([vp], C.c_int),
+        "sigp_mlp_get_params": ([vp, vp], C.c_int),
+        "sigp_mlp_set_params": ([vp, vp], C.c_int),
+        "sigp_mlp_gradients": ([vp, vp, vp, i32, vp, vp], C.c_int),
+        "sigp_mlp_train": ([vp, vp, vp, i64, i32, i32, C.c_float, C.c_uint64, vp], C.c_int),
+        "sigp_mlp_predict": ([vp, vp, i64, vp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(L, name)
