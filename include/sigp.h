@@ -146,7 +146,8 @@ sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst, con
  *     take the first maximum together.  For few scenarios per rule, such as
  *     the reference's GP loop, which evaluates one per generation
  *     (engine/gp.hpp:32).
- *   SIGP_MAP_AUTO: candidate lanes when T < 16, scenario lanes otherwise.
+ *   SIGP_MAP_AUTO: candidate lanes when T < 16 or when scenario lanes would
+ *     make fewer than 512 workgroups, scenario lanes otherwise.
  * Each wavefront keeps the candidates' terminal values and the postfix stack
  * in LDS, (distinct slots read + deepest stack - 1) * 512 bytes, and a
  * workgroup stays within 64 KiB of LDS: a batch whose rules need more runs
@@ -241,6 +242,75 @@ sigp_status sigp_mlp_train(sigp_mlp *net, const float *x, const float *t, int64_
                            int32_t epochs, int32_t batch, float lr, uint64_t shuffle_seed,
                            float *epoch_loss);
 sigp_status sigp_mlp_predict(sigp_mlp *net, const float *x, int64_t rows, float *y_out);
+
+/* ------------------------------------------------------------------------
+ * The GP loop on the device.  Replaces the host loop of the reference's
+ * generation engine -- evolve_one (engine/gp_engine.cu:66-84) and the
+ * seed-batched exhaust_batched (engine/gp_engine.cu:131-183) -- whose step
+ * is gp_step (engine/gp.hpp:170-194) over ramped_half_and_half
+ * (engine/gp.hpp:67-77).  Populations stay on the device as prefix-order
+ * token arrays; every random choice comes from one CPython-compatible
+ * Mersenne Twister per run seeded like Mt19937(seed) (engine/rng.hpp), drawn
+ * in the reference's order, so the populations are the reference's.
+ *
+ * Trees are at most SIGP_GP_MAX_NODES tokens (depth <= 8): a terminal is its
+ * index in terminal_slots, a function SIGP_GP_TOK_FUNC | its index in
+ * function_ops, an ERC constant SIGP_GP_TOK_CONST | k for the value k/1000
+ * (the reference's "%.3f" label as compile reads it back). */
+#define SIGP_GP_MAX_NODES 512
+#define SIGP_GP_TOK_FUNC 0x4000
+#define SIGP_GP_TOK_CONST 0x2000
+
+typedef struct sigp_gp sigp_gp;
+
+/* TreeConfig + EvolutionConfig (engine/gp.hpp:13-34). */
+typedef struct {
+    int32_t n_runs;                  /* independent runs evolved side by side (the reference's seeds) */
+    int32_t population;              /* EvolutionConfig::population_size, <= 6144 */
+    int32_t offspring;               /* offspring per run and step: 0 = population */
+    int32_t n_terminals;             /* TreeConfig::terminals, <= 128 */
+    const int32_t *terminal_slots;   /* slot of each terminal (0..60), -1 = "ERC" */
+    int32_t n_functions;             /* TreeConfig::functions, <= 8 */
+    const int32_t *function_ops;     /* opcode of each function (0..6, engine/vocab.hpp:79-88) */
+    int32_t min_depth, max_depth;    /* TreeConfig: initialisation and mutation depths; mutants deeper than max_depth are rejected */
+    int32_t depth_limit;             /* EvolutionConfig::max_depth: crossover children deeper are rejected (<= 8) */
+    int32_t tournament_size;         /* <= 64 */
+    int32_t elitism;
+    double crossover_rate, mutation_rate;
+    int32_t T;                       /* training scenarios per run and generation */
+    sigp_launch_config launch;       /* evaluator lane mapping */
+} sigp_gp_config;
+
+/* seeds: n_runs Mt19937 seeds; term_mask: int64[SIGP_N_TERMS] of the
+ * evaluation (engine/gp_engine.cu:93). */
+sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config *cfg, const uint64_t *seeds,
+                           const int64_t *term_mask, sigp_gp **out);
+sigp_status sigp_gp_destroy(sigp_gp *gp);
+/* ramped_half_and_half of every run (engine/gp.hpp:67-77). */
+sigp_status sigp_gp_init(sigp_gp *gp);
+/* Fitness of the population (engine/gp_engine.cu:70-79): run r's rules read
+ * host scenario rows r*T .. r*T+T-1 (n_rows = n_runs*T rows of n durations);
+ * fitness (nullable): n_runs*population means. */
+sigp_status sigp_gp_evaluate(sigp_gp *gp, const double *sampled, int64_t n_rows, double *fitness);
+/* One gp_step of every run (engine/gp.hpp:170-194) with the given host
+ * fitness (n_runs*population), or NULL for the last sigp_gp_evaluate. */
+sigp_status sigp_gp_step(sigp_gp *gp, const double *fitness);
+/* The whole loop on the device: for g < generations, evaluate on rows
+ * g*n_runs*T .. (n_rows = generations*n_runs*T) and step, except after the
+ * last.  use_graph != 0 captures one generation as a HIP graph and replays
+ * it.  fitness_history (nullable): generations x n_runs x population;
+ * elapsed_ms (nullable): device time of the loop. */
+sigp_status sigp_gp_run(sigp_gp *gp, const double *sampled, int64_t n_rows, int32_t generations,
+                        int32_t use_graph, double *fitness_history, double *elapsed_ms);
+/* The population: tokens (n_runs*population*SIGP_GP_MAX_NODES) and lengths. */
+sigp_status sigp_gp_population(const sigp_gp *gp, int16_t *tokens, int32_t *lengths);
+/* The population compiled as pack_trees would (engine/factory.cuh:20-40),
+ * tree i at i*SIGP_GP_MAX_NODES: ops (n_runs*population*SIGP_GP_MAX_NODES),
+ * args (2x that), lengths. */
+sigp_status sigp_gp_packed(const sigp_gp *gp, int32_t *ops, int32_t *args, int32_t *lengths);
+/* Generator state of one run: 624 words and the position, as CPython's
+ * random.getstate()[1]. */
+sigp_status sigp_gp_rng_state(const sigp_gp *gp, int32_t run, uint32_t *state);
 
 #ifdef __cplusplus
 }

@@ -20,26 +20,9 @@
 // padded the same way.
 #pragma once
 #include "sigp_terms.h"
+#include "sigp_rules.h"
 
 namespace sigp {
-
-// Per-rule computation flags (a second 64-bit word beside the slot bits).
-enum : uint64_t {
-    F_NS = 1ull << 0,
-    F_REGRET = 1ull << 1,
-    F_NSROB = 1ull << 2,
-    F_MAXN = 1ull << 3,
-    F_REACH2 = 1ull << 4,
-    F_NSTW = 1ull << 5,
-    F_STRUCT = 1ull << 6,
-    F_REACHK = 1ull << 7,    // greedy path REACH1/3/5 (length in bits 40-42)
-    F_ATRISK = 1ull << 8,
-    F_R2ROB = 1ull << 9,
-    F_REACHKR = 1ull << 10,  // robust path REACH3ROB/5ROB (length in bits 43-45)
-    F_FNOWROB = 1ull << 11,  // per-decision robust bits (NFEASROB)
-    F_FSCORE = 1ull << 12,   // per-decision FEASCORE sum
-    F_CNOW = 1ull << 13,     // per-decision close-only bits
-};
 
 struct FusedTabs {
     int ldp;   // row length of [a][j] tables (>= n, multiple of 8); rows of [j][a] tables

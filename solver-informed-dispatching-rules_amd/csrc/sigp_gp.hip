@@ -1,0 +1,1013 @@
+// The GP loop on the device (engine/gp.hpp, engine/gp_engine.cu:66-84 and
+// :131-183): populations, breeding and evaluation stay in HBM, and one
+// generation is a fixed sequence of kernels that a HIP graph captures once.
+//
+// Trees.  Each tree is a prefix-order array of 16-bit tokens in a slot of
+// SIGP_GP_MAX_NODES (a binary tree of depth <= 8 has at most 511 nodes):
+// a terminal is its index in the run's terminal list, a function is
+// TOK_FUNC | its index in the function list, an ERC constant is
+// TOK_CONST | k for the value k / 1000.  Every generation the parents are
+// analysed by one wavefront per tree: node depth, subtree size and height,
+// prefix/suffix maxima of the depth, the pre-order positions of the internal
+// nodes and of the leaves -- which makes every question breeding asks of a
+// parent (Koza point choice, the subtree it roots, the depth of the child)
+// an O(1) lookup -- and compiled to the evaluator's postfix arrays by a
+// parallel scatter: node k of pre-order goes to post-order position
+// k - depth(k) + size(k) - 1.
+//
+// Breeding.  The reference draws every random choice of a generation from
+// one CPython-compatible Mersenne Twister per run, in a fixed sequential
+// order.  One wavefront per run executes that sequence: all 64 lanes run the
+// same scalar program in lockstep (identical values, no divergence), lane 0
+// stores the results, and when the generator's 624 words are used up the
+// lanes regenerate them together in LDS.  The wavefront writes one recipe per
+// offspring (copy / crossover of two parents at two points / mutation with a
+// freshly grown subtree); a second kernel assembles all offspring in
+// parallel.  Draws and outcomes equal engine/gp.hpp step for step, so the
+// populations are identical to the reference's (verify/golden/gp.txt).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sigp.h"
+#include "sigp_internal.h"
+#include "sigp_rules.h"
+
+using namespace sigp;
+
+#define GP_TRY(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            return sigp_internal_fail(e_ == hipErrorOutOfMemory ? SIGP_OOM : SIGP_DEVICE_ERROR,     \
+                                      std::string(#x) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+namespace {
+
+constexpr int MAXL = SIGP_GP_MAX_NODES;
+constexpr int MAX_DEPTH = 8;          // trees of depth <= 8 fit MAXL
+constexpr int MAX_TERMS = 128;
+constexpr int MAX_FUNCS = 8;
+constexpr int MAX_POOL = 512;         // sample()'s pool method: population <= setsize
+constexpr int MAX_TOURN = 64;
+constexpr int MAX_POP = 6144;         // the breeding wavefront keeps the fitness in LDS
+constexpr int TOK_FUNC = SIGP_GP_TOK_FUNC;
+constexpr int TOK_CONST = SIGP_GP_TOK_CONST;
+constexpr int MT_WORDS = 625;         // 624 state words + position
+
+struct GpParams {
+    int nt, nf;
+    int tslot[MAX_TERMS];   // slot of each terminal, -1 = ERC
+    int fop[MAX_FUNCS];     // opcode of each function
+    int min_depth, max_depth, depth_limit;
+    int tourn, elitism;
+    int pop;                // parents per run
+    int n_off;              // offspring per run and step
+    int setsize;            // CPython sample(): pool method when pop <= setsize
+    double p_cx, p_cxmut;   // crossover_rate, crossover_rate + mutation_rate
+    double p_term;          // n_terminals / (n_terminals + n_functions)
+};
+
+enum { R_COPY = 0, R_XOVER = 1, R_MUT = 2 };
+// Offspring o of a run: parent a (within the run), and for crossover the
+// point pa in a replaced by b's subtree at pb; for mutation by the glen
+// tokens grown into the offspring's grow slot.
+struct Recipe {
+    int kind, a, pa, b, pb, glen;
+};
+
+// A set of trees and what their analysis writes.  Parents need the breeding
+// tables (size .. nint); every evaluated set needs the compiled arrays.
+struct TreeSet {
+    const int16_t *tok;
+    const int32_t *len;
+    int count;
+    int16_t *size, *pos;                      // per node; pos: internals, then leaves
+    int8_t *depth, *height, *pmax, *smax;     // per node
+    int32_t *nint;                            // per tree
+    int32_t *raw_ops, *raw_args;              // pack_trees format (engine/factory.cuh:20-40)
+    int32_t *ops, *args, *off, *plen;         // prepared for the kernels
+    uint64_t *rd, *flags;
+    float *cost;
+};
+
+// ---------------------------------------------------------------------------
+// CPython's Mersenne Twister (engine/rng.hpp:54-93), state in LDS, driven by
+// the whole wavefront in lockstep.
+
+// Regenerate the 624 words.  Word i reads words i + 1 (old) and i + 397
+// (old for i < 227) or i - 227 (already new): in chunks of 64 ascending
+// indices every such new word lies in an earlier chunk, so the chunks run
+// lane-parallel with loads and stores separated by barriers.
+__device__ void mt_twist(uint32_t *mt) {
+    const int lane = (int)threadIdx.x;
+    for (int base = 0; base < 623; base += 64) {
+        const int i = base + lane;
+        uint32_t nv = 0;
+        if (i < 623) {
+            const uint32_t y = (mt[i] & 0x80000000u) | (mt[i + 1] & 0x7fffffffu);
+            nv = mt[i < 227 ? i + 397 : i - 227] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        __syncthreads();
+        if (i < 623) mt[i] = nv;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        const uint32_t y = (mt[623] & 0x80000000u) | (mt[0] & 0x7fffffffu);
+        mt[623] = mt[396] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    __syncthreads();
+}
+
+struct Rng {
+    uint32_t *mt;
+    int idx;
+
+    __device__ uint32_t next() {
+        if (idx >= 624) {
+            mt_twist(mt);
+            idx = 0;
+        }
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        return y;
+    }
+    // the 53-bit integer m of random() = m / 2^53
+    __device__ uint64_t bits53() {
+        const uint64_t a = next() >> 5, b = next() >> 6;
+        return (a << 26) | b;
+    }
+    __device__ double random() { return (double)bits53() * (1.0 / 9007199254740992.0); }
+    // randbelow: rejection on bit_length(n) top bits; 0 for n == 0
+    __device__ uint32_t below(uint32_t n) {
+        if (n == 0) return 0;
+        const int k = 32 - __clz(n);
+        uint32_t r = next() >> (32 - k);
+        while (r >= n) r = next() >> (32 - k);
+        return r;
+    }
+};
+
+// The ERC label "%.3f" % random() read back by compile (llround(v * 1000),
+// engine/vocab.hpp:101-104) is k = m * 1000 / 2^53 rounded to nearest, ties
+// to even (how printf rounds an exactly representable tie) -- computed
+// exactly in 64-bit integers.
+__device__ int erc_thousandths(uint64_t m) {
+    const uint64_t x = m * 1000ull;               // < 2^63
+    uint64_t q = x >> 53;
+    const uint64_t r = x & ((1ull << 53) - 1);
+    const uint64_t half = 1ull << 52;
+    if (r > half || (r == half && (q & 1))) ++q;
+    return (int)q;
+}
+
+// engine/gp.hpp:37-45
+__device__ int pick_terminal(const GpParams &P, Rng &g) {
+    const int t = (int)g.below((uint32_t)P.nt);
+    if (P.tslot[t] < 0) return TOK_CONST | erc_thousandths(g.bits53());
+    return t;
+}
+
+// engine/gp.hpp:47-64 (generate_full, generate_grow): the recursion as a
+// stack of pending depths, which emits nodes and makes draws in the same
+// pre-order.  Lane 0 stores the tokens; returns the count and the height.
+__device__ int generate(const GpParams &P, Rng &g, int *gst, int depth, bool full, int16_t *out,
+                        int &height) {
+    int sp = 0, cnt = 0;
+    height = 0;
+    gst[sp++] = depth;
+    while (sp > 0) {
+        const int d = gst[--sp];
+        bool term = d <= 0;
+        if (!term && !full) term = g.random() < P.p_term;
+        int tk;
+        if (term) {
+            tk = pick_terminal(P, g);
+            height = max(height, depth - d);
+        } else {
+            tk = TOK_FUNC | (int)g.below((uint32_t)P.nf);
+            gst[sp++] = d - 1;
+            gst[sp++] = d - 1;
+        }
+        if (threadIdx.x == 0) out[cnt] = (int16_t)tk;
+        ++cnt;
+    }
+    return cnt;
+}
+
+__device__ void load_rng(Rng &g, uint32_t *mt, const uint32_t *state) {
+    for (int k = (int)threadIdx.x; k < 624; k += 64) mt[k] = state[k];
+    __syncthreads();
+    g.mt = mt;
+    g.idx = (int)state[624];
+}
+
+__device__ void store_rng(const Rng &g, uint32_t *state) {
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < 624; k += 64) state[k] = g.mt[k];
+    if (threadIdx.x == 0) state[624] = (uint32_t)g.idx;
+}
+
+// engine/gp.hpp:67-77: depths ramp min..max, alternating full/grow blocks.
+// One wavefront per run.
+__global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, int16_t *tok, int32_t *len) {
+    __shared__ uint32_t mt[624];
+    __shared__ int gst[2 * MAX_DEPTH + 4];
+    const int r = (int)blockIdx.x;
+    Rng g;
+    load_rng(g, mt, state + (size_t)r * MT_WORDS);
+    const int nd = P.max_depth - P.min_depth + 1;
+    for (int i = 0; i < P.pop; ++i) {
+        const int d = P.min_depth + i % nd;
+        const bool full = (i / nd) % 2 == 0;
+        int h;
+        const size_t gi = (size_t)r * P.pop + i;
+        const int n = generate(P, g, gst, d, full, tok + gi * MAXL, h);
+        if (threadIdx.x == 0) len[gi] = n;
+    }
+    store_rng(g, state + (size_t)r * MT_WORDS);
+}
+
+// ---------------------------------------------------------------------------
+// breeding (engine/gp.hpp:118-194)
+
+struct Parents {
+    const int32_t *len, *nint;
+    const int16_t *size, *pos;
+    const int8_t *depth, *height, *pmax, *smax;
+};
+
+// engine/gp.hpp:120-131: Koza 90/10; the draw of the class happens only
+// when the tree has internal nodes (it always has leaves).
+__device__ int select_point(const Parents &T, size_t gi, Rng &g) {
+    const int L = T.len[gi], ni = T.nint[gi], nl = L - ni;
+    const int16_t *pos = T.pos + gi * MAXL;
+    if (ni > 0 && (nl == 0 || g.random() < 0.9)) return pos[g.below((uint32_t)ni)];
+    if (nl > 0) return pos[ni + (int)g.below((uint32_t)nl)];
+    return pos[g.below((uint32_t)ni)];
+}
+
+// Depth of the nodes of tree gi outside the subtree rooted at p (-1: none).
+__device__ int outside_depth(const Parents &T, size_t gi, int p) {
+    const int L = T.len[gi];
+    const int e = p + T.size[gi * MAXL + p];
+    int d = p > 0 ? T.pmax[gi * MAXL + p - 1] : -1;
+    if (e < L) d = max(d, (int)T.smax[gi * MAXL + e]);
+    return d;
+}
+
+// engine/gp.hpp:157-166 with engine/rng.hpp sample(): k distinct
+// contenders, the first maximum in contender order wins.
+__device__ int tournament(const GpParams &P, Rng &g, const double *fit, int *res, int *pool) {
+    const int n = P.pop, k = min(P.tourn, n);
+    if (n <= P.setsize) {
+        for (int j = (int)threadIdx.x; j < n; j += 64) pool[j] = j;
+        __syncthreads();
+        for (int i = 0; i < k; ++i) {
+            const int j = (int)g.below((uint32_t)(n - i));
+            const int pj = pool[j];
+            const int last = pool[n - i - 1];
+            __syncthreads();
+            res[i] = pj;
+            pool[j] = last;
+            __syncthreads();
+        }
+    } else {
+        for (int i = 0; i < k; ++i) {
+            int j = (int)g.below((uint32_t)n);
+            for (;;) {
+                bool seen = false;
+                for (int q = 0; q < i; ++q) seen |= res[q] == j;
+                if (!seen) break;
+                j = (int)g.below((uint32_t)n);
+            }
+            res[i] = j;
+        }
+    }
+    int best = res[0];
+    for (int i = 1; i < k; ++i)
+        if (fit[res[i]] > fit[best]) best = res[i];
+    return best;
+}
+
+__device__ void put_recipe(Recipe *rec, int o, int kind, int a, int pa, int b, int pb, int glen) {
+    if (threadIdx.x == 0) {
+        Recipe q;
+        q.kind = kind;
+        q.a = a;
+        q.pa = pa;
+        q.b = b;
+        q.pb = pb;
+        q.glen = glen;
+        rec[o] = q;
+    }
+}
+
+// One generation step of every run (engine/gp.hpp:170-194): the elites in
+// the order of the reference's stable descending sort, then crossover (two
+// children, the second only while room remains), mutation or reproduction by
+// rate until n_off offspring exist.  One wavefront per run.
+__global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, const double *fitness,
+                                                  Parents T, Recipe *recipes, int16_t *grow) {
+    __shared__ uint32_t mt[624];
+    __shared__ int pool[MAX_POOL];
+    __shared__ int res[MAX_TOURN];
+    __shared__ int gst[2 * MAX_DEPTH + 4];
+    extern __shared__ double sh[];
+    double *fit = sh;                                   // pop
+    unsigned char *taken = (unsigned char *)(sh + P.pop);   // pop
+    const int r = (int)blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    const size_t base = (size_t)r * P.pop;
+    Recipe *rec = recipes + (size_t)r * P.n_off;
+    for (int j = lane; j < P.pop; j += 64) {
+        fit[j] = fitness[base + j];
+        taken[j] = 0;
+    }
+    Rng g;
+    load_rng(g, mt, state + (size_t)r * MT_WORDS);
+
+    int o = 0;
+    const int E = min(P.elitism, min(P.pop, P.n_off));
+    for (int e = 0; e < E; ++e) {
+        double bf = 0.0;
+        int bi = -1;
+        for (int j = lane; j < P.pop; j += 64)
+            if (!taken[j] && (bi < 0 || fit[j] > bf)) {
+                bf = fit[j];
+                bi = j;
+            }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const double of = __shfl_xor(bf, off);
+            const int oi = __shfl_xor(bi, off);
+            if (oi >= 0 && (bi < 0 || of > bf || (of == bf && oi < bi))) {
+                bf = of;
+                bi = oi;
+            }
+        }
+        __syncthreads();
+        if (lane == 0) taken[bi] = 1;
+        __syncthreads();
+        put_recipe(rec, o++, R_COPY, bi, 0, 0, 0, 0);
+    }
+    while (o < P.n_off) {
+        const double u = g.random();
+        if (u < P.p_cx) {
+            const int a = tournament(P, g, fit, res, pool);
+            const int b = tournament(P, g, fit, res, pool);
+            const size_t ga = base + a, gb = base + b;
+            const int pa = select_point(T, ga, g);
+            const int pb = select_point(T, gb, g);
+            const int da = T.depth[ga * MAXL + pa], db = T.depth[gb * MAXL + pb];
+            const int ha = T.height[ga * MAXL + pa], hb = T.height[gb * MAXL + pb];
+            const int d1 = max(outside_depth(T, ga, pa), da + hb);
+            const int d2 = max(outside_depth(T, gb, pb), db + ha);
+            if (d1 > P.depth_limit) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0);
+            else put_recipe(rec, o, R_XOVER, a, pa, b, pb, 0);
+            ++o;
+            if (o < P.n_off) {
+                if (d2 > P.depth_limit) put_recipe(rec, o, R_COPY, b, 0, 0, 0, 0);
+                else put_recipe(rec, o, R_XOVER, b, pb, a, pa, 0);
+                ++o;
+            }
+        } else if (u < P.p_cxmut) {
+            // engine/gp.hpp:146-153
+            const int a = tournament(P, g, fit, res, pool);
+            const size_t ga = base + a;
+            const int pa = select_point(T, ga, g);
+            const int nd = P.min_depth + (int)g.below((uint32_t)(P.max_depth - P.min_depth + 1));
+            int h;
+            const int glen = generate(P, g, gst, nd, false, grow + ((size_t)r * P.n_off + o) * MAXL, h);
+            const int d = max(outside_depth(T, ga, pa), (int)T.depth[ga * MAXL + pa] + h);
+            if (d > P.max_depth) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0);
+            else put_recipe(rec, o, R_MUT, a, pa, 0, 0, glen);
+            ++o;
+        } else {
+            put_recipe(rec, o, R_COPY, tournament(P, g, fit, res, pool), 0, 0, 0, 0);
+            ++o;
+        }
+    }
+    store_rng(g, state + (size_t)r * MT_WORDS);
+}
+
+// Offspring from recipes, one wavefront each: the prefix of parent a up to
+// the point, the donor subtree, the rest of a.
+__global__ __launch_bounds__(64) void assemble_kernel(GpParams P, const Recipe *recipes, const int16_t *ptok,
+                                                      Parents T, const int16_t *grow, int16_t *otok,
+                                                      int32_t *olen) {
+    const int r = (int)blockIdx.y, o = (int)blockIdx.x;
+    const size_t slot = (size_t)r * P.n_off + o;
+    const Recipe q = recipes[slot];
+    const size_t ga = (size_t)r * P.pop + q.a;
+    const int16_t *A = ptok + ga * MAXL;
+    int16_t *out = otok + slot * MAXL;
+    const int La = T.len[ga];
+    int L = La, pa = La, sa = 0, pb = 0, sb = 0;
+    const int16_t *Bt = A;
+    if (q.kind == R_XOVER) {
+        const size_t gb = (size_t)r * P.pop + q.b;
+        pa = q.pa;
+        pb = q.pb;
+        sa = T.size[ga * MAXL + pa];
+        sb = T.size[gb * MAXL + pb];
+        Bt = ptok + gb * MAXL;
+    } else if (q.kind == R_MUT) {
+        pa = q.pa;
+        sa = T.size[ga * MAXL + pa];
+        sb = q.glen;
+        Bt = grow + slot * MAXL;
+    }
+    L = La - sa + sb;   // <= 511: the depth checks kept every child within depth 8
+    for (int k = (int)threadIdx.x; k < L; k += 64)
+        out[k] = k < pa ? A[k] : (k < pa + sb ? Bt[pb + k - pa] : A[k - sb + sa]);
+    if (threadIdx.x == 0) olen[slot] = L;
+}
+
+// The next parents: tree i of run r is offspring sel[r * pop + i] of the
+// run (identity when sel is null).
+__global__ __launch_bounds__(64) void gather_kernel(GpParams P, const int32_t *sel, const int16_t *otok,
+                                                    const int32_t *olen, int16_t *ptok, int32_t *plen) {
+    const int r = (int)blockIdx.y, i = (int)blockIdx.x;
+    const size_t dst = (size_t)r * P.pop + i;
+    const size_t src = (size_t)r * P.n_off + (sel ? sel[dst] : i);
+    const int L = olen[src];
+    for (int k = (int)threadIdx.x; k < L; k += 64) ptok[dst * MAXL + k] = otok[src * MAXL + k];
+    if (threadIdx.x == 0) plen[dst] = L;
+}
+
+// Per tree, one wavefront: the breeding tables (when T.size is set) and the
+// compiled postfix arrays (engine/vocab.hpp:96-125 compile_rmod, and the
+// batch preparation of the evaluator: dead leaves become the constant 0,
+// live ones index the compact value store).
+__global__ __launch_bounds__(64) void analyze_kernel(GpParams P, TreeSet T, uint64_t mask_bits) {
+    __shared__ int16_t stok[MAXL], ssize[MAXL];
+    __shared__ int8_t sdep[MAXL], shei[MAXL];
+    __shared__ int stk_pos[MAX_DEPTH + 4], stk_rem[MAX_DEPTH + 4], stk_h[MAX_DEPTH + 4];
+    __shared__ uint64_t s_used;
+    __shared__ int s_nint;
+    const size_t gi = blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int L = T.len[gi];
+    for (int k = lane; k < L; k += 64) stok[k] = T.tok[gi * MAXL + k];
+    __syncthreads();
+    if (lane == 0) {
+        int sp = 0, nint = 0;
+        uint64_t used = 0;
+        for (int k = 0; k < L; ++k) {
+            const int t = stok[k];
+            sdep[k] = (int8_t)sp;
+            if (t & TOK_FUNC) {
+                stk_pos[sp] = k;
+                stk_rem[sp] = 2;
+                stk_h[sp] = 0;
+                ++sp;
+                ++nint;
+                continue;
+            }
+            if (!(t & TOK_CONST)) used |= bit(P.tslot[t]);
+            ssize[k] = 1;
+            shei[k] = 0;
+            int h = 0;
+            while (sp > 0) {
+                const int top = sp - 1;
+                stk_h[top] = max(stk_h[top], h + 1);
+                if (--stk_rem[top] > 0) break;
+                const int q = stk_pos[top];
+                ssize[q] = (int16_t)(k - q + 1);
+                shei[q] = (int8_t)stk_h[top];
+                h = stk_h[top];
+                --sp;
+            }
+        }
+        s_used = used;
+        s_nint = nint;
+    }
+    __syncthreads();
+    const int nint = s_nint;
+    if (T.size) {
+        if (lane == 0) {
+            int m = -1;
+            for (int k = 0; k < L; ++k) {
+                m = max(m, (int)sdep[k]);
+                T.pmax[gi * MAXL + k] = (int8_t)m;
+            }
+            m = -1;
+            for (int k = L - 1; k >= 0; --k) {
+                m = max(m, (int)sdep[k]);
+                T.smax[gi * MAXL + k] = (int8_t)m;
+            }
+            T.nint[gi] = nint;
+        }
+        for (int k = lane; k < L; k += 64) {
+            T.size[gi * MAXL + k] = ssize[k];
+            T.depth[gi * MAXL + k] = sdep[k];
+            T.height[gi * MAXL + k] = shei[k];
+        }
+        // pre-order positions: internal nodes first, then leaves, each ascending
+        int ci = 0, cl = 0;
+        const uint64_t below_me = (1ull << lane) - 1;
+        for (int b = 0; b < L; b += 64) {
+            const int k = b + lane;
+            const bool in = k < L;
+            const bool isf = in && (stok[k] & TOK_FUNC);
+            const uint64_t mf = __ballot(isf), ml = __ballot(in && !isf);
+            if (isf) T.pos[gi * MAXL + ci + __popcll(mf & below_me)] = (int16_t)k;
+            if (in && !isf) T.pos[gi * MAXL + nint + cl + __popcll(ml & below_me)] = (int16_t)k;
+            ci += __popcll(mf);
+            cl += __popcll(ml);
+        }
+    }
+    uint64_t rd, comp, fl;
+    compute_sets(s_used, mask_bits, &rd, &comp, &fl);
+    const size_t o = gi * MAXL;
+    for (int k = lane; k < L; k += 64) {
+        const int t = stok[k];
+        const int post = k - sdep[k] + ssize[k] - 1;
+        int rop, ra0, ra1, pop_, pa0;
+        if (t & TOK_FUNC) {
+            rop = P.fop[t & 0xff];
+            ra0 = k - sdep[k] + ssize[k + 1] - 1;   // left child k + 1 in post-order
+            ra1 = post - 1;                          // right child ends just before its parent
+            pop_ = rop;
+            pa0 = ra0;
+        } else if (t & TOK_CONST) {
+            rop = -2;
+            ra0 = t & (TOK_CONST - 1);
+            ra1 = 0;
+            pop_ = -2;
+            pa0 = ra0;
+        } else {
+            const int s = P.tslot[t];
+            rop = -1;
+            ra0 = s;
+            ra1 = 0;
+            if (rd & bit(s)) {
+                pop_ = -1;
+                pa0 = __popcll(rd & (bit(s) - 1));
+            } else {
+                pop_ = -2;   // a dead slot reads 0.0
+                pa0 = 0;
+            }
+        }
+        if (T.raw_ops) {
+            T.raw_ops[o + post] = rop;
+            T.raw_args[2 * (o + post)] = ra0;
+            T.raw_args[2 * (o + post) + 1] = ra1;
+        }
+        T.ops[o + post] = pop_;
+        T.args[2 * (o + post)] = pa0;
+        T.args[2 * (o + post) + 1] = (t & TOK_FUNC) ? ra1 : 0;
+    }
+    if (lane == 0) {
+        T.off[gi] = (int32_t)o;
+        T.plen[gi] = L;
+        T.rd[gi] = rd;
+        T.flags[gi] = fl;
+        T.cost[gi] = rule_cost(L, fl, comp);
+    }
+}
+
+// Launch order: rules by cost, descending, ties by index (a stable sort),
+// by counting for each rule the rules that precede it.
+__global__ __launch_bounds__(256) void order_kernel(const float *cost, int P, int32_t *order) {
+    __shared__ float tile[1024];
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const float ci = i < P ? cost[i] : 0.0f;
+    int rank = 0;
+    for (int b = 0; b < P; b += 1024) {
+        for (int j = (int)threadIdx.x; j < 1024; j += (int)blockDim.x) tile[j] = b + j < P ? cost[b + j] : 0.0f;
+        __syncthreads();
+        const int m = min(1024, P - b);
+        for (int j = 0; j < m; ++j) {
+            const float cj = tile[j];
+            rank += (cj > ci || (cj == ci && b + j < i)) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+    if (i < P) order[rank] = i;
+}
+
+// Generation counter of a captured loop and the per-generation copies that
+// read it (a graph replays with fixed arguments).
+__global__ void rows_kernel(const double *all, const int *gen, long long per_gen, double *dst) {
+    const long long g = *gen;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < per_gen;
+         k += (long long)gridDim.x * blockDim.x)
+        dst[k] = all[g * per_gen + k];
+}
+
+__global__ void record_kernel(const double *fit, const int *gen, int P, double *hist) {
+    const long long g = *gen;
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < P) hist[g * P + i] = fit[i];
+}
+
+__global__ void tick_kernel(int *gen) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *gen += 1;
+}
+
+__global__ void set_kernel(int *gen, int v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *gen = v;
+}
+
+// CPython's seeding of an integer (engine/rng.hpp:21-52 init_by_array with
+// the 32-bit words of the seed, least significant first).
+void mt_seed(uint64_t seed, uint32_t *st) {
+    std::vector<uint32_t> key;
+    if (seed == 0) key.push_back(0);
+    while (seed) {
+        key.push_back((uint32_t)(seed & 0xffffffffu));
+        seed >>= 32;
+    }
+    uint32_t *mt = st;
+    mt[0] = 19650218u;
+    for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+    size_t i = 1, j = 0;
+    for (size_t k = std::max<size_t>(624, key.size()); k; --k) {
+        mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
+        ++i;
+        ++j;
+        if (i >= 624) {
+            mt[0] = mt[623];
+            i = 1;
+        }
+        if (j >= key.size()) j = 0;
+    }
+    for (size_t k = 623; k; --k) {
+        mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
+        ++i;
+        if (i >= 624) {
+            mt[0] = mt[623];
+            i = 1;
+        }
+    }
+    mt[0] = 0x80000000u;
+    st[624] = 624;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host object
+
+struct sigp_gp {
+    sigp_instance *inst = nullptr;
+    GpParams prm{};
+    int runs = 0, T = 0, n = 0;
+    uint64_t mask = 0;
+    int P = 0;                        // runs x pop
+    char *dbuf = nullptr;             // every device array below
+    uint32_t *mt = nullptr;           // runs x 625
+    int16_t *ptok = nullptr, *otok = nullptr, *grow = nullptr;
+    int32_t *plen = nullptr, *olen = nullptr;
+    Recipe *rec = nullptr;
+    TreeSet pset{};                   // the parents: tables + compiled arrays
+    int32_t *order = nullptr;
+    double *fit = nullptr, *pairs = nullptr, *scen = nullptr;
+    int *gen = nullptr;
+    sigp_batch *bt = nullptr;
+    hipStream_t stream = nullptr;
+    bool ready = false;               // a population exists
+};
+
+static sigp_status gp_fail(const std::string &m) { return sigp_internal_fail(SIGP_INVALID_ARG, m); }
+
+static Parents parents_of(const sigp_gp *g) {
+    Parents T;
+    T.len = g->pset.len;
+    T.nint = g->pset.nint;
+    T.size = g->pset.size;
+    T.pos = g->pset.pos;
+    T.depth = g->pset.depth;
+    T.height = g->pset.height;
+    T.pmax = g->pset.pmax;
+    T.smax = g->pset.smax;
+    return T;
+}
+
+// Analysis, compilation and launch order of the parents.
+static void launch_rebuild(sigp_gp *g) {
+    analyze_kernel<<<g->P, 64, 0, g->stream>>>(g->prm, g->pset, g->mask);
+    order_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->pset.cost, g->P, g->order);
+}
+
+static void launch_step(sigp_gp *g) {
+    const size_t lds = (size_t)g->prm.pop * (sizeof(double) + 1);
+    plan_kernel<<<g->runs, 64, lds, g->stream>>>(g->prm, g->mt, g->fit, parents_of(g), g->rec, g->grow);
+    assemble_kernel<<<dim3(g->prm.n_off, g->runs), 64, 0, g->stream>>>(g->prm, g->rec, g->ptok, parents_of(g),
+                                                                        g->grow, g->otok, g->olen);
+    gather_kernel<<<dim3(g->prm.pop, g->runs), 64, 0, g->stream>>>(g->prm, nullptr, g->otok, g->olen, g->ptok,
+                                                                    g->plen);
+    launch_rebuild(g);
+}
+
+extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config *c, const uint64_t *seeds,
+                                      const int64_t *term_mask, sigp_gp **out) {
+    if (!out) return gp_fail("out is NULL");
+    *out = nullptr;
+    if (!inst || !c || !seeds || !term_mask) return gp_fail("NULL argument");
+    if (c->n_runs < 1 || c->n_runs > 65535) return gp_fail("n_runs must be in 1..65535");
+    if (c->population < 1 || c->population > MAX_POP)
+        return gp_fail("population must be in 1.." + std::to_string(MAX_POP));
+    const int n_off = c->offspring == 0 ? c->population : c->offspring;
+    if (n_off != c->population)
+        return gp_fail("offspring must equal population (0) for the plain GP step");
+    if (c->n_terminals < 1 || c->n_terminals > MAX_TERMS || !c->terminal_slots)
+        return gp_fail("need 1.." + std::to_string(MAX_TERMS) + " terminals");
+    if (c->n_functions < 1 || c->n_functions > MAX_FUNCS || !c->function_ops)
+        return gp_fail("need 1.." + std::to_string(MAX_FUNCS) + " functions");
+    if (c->min_depth < 0 || c->min_depth > c->max_depth || c->max_depth > MAX_DEPTH || c->depth_limit < 0 ||
+        c->depth_limit > MAX_DEPTH)
+        return gp_fail("need 0 <= min_depth <= max_depth <= 8 and 0 <= depth_limit <= 8");
+    if (c->tournament_size < 1 || c->tournament_size > MAX_TOURN) return gp_fail("tournament_size must be in 1..64");
+    if (c->elitism < 0) return gp_fail("elitism < 0");
+    if (c->T < 1) return gp_fail("T must be >= 1");
+    if ((long long)c->n_runs * c->population > (1 << 20)) return gp_fail("n_runs x population exceeds 2^20");
+
+    GpParams p{};
+    p.nt = c->n_terminals;
+    p.nf = c->n_functions;
+    uint64_t vocab = 0;
+    for (int t = 0; t < p.nt; ++t) {
+        const int s = c->terminal_slots[t];
+        if (s < -1 || s >= SIGP_N_TERMS) return gp_fail("terminal slot out of range");
+        p.tslot[t] = s;
+        if (s >= 0) vocab |= bit(s);
+    }
+    bool rmod = false;
+    for (int f = 0; f < p.nf; ++f) {
+        const int op = c->function_ops[f];
+        if (op < 0 || op > 6) return gp_fail("function opcode out of range");
+        p.fop[f] = op;
+        rmod |= op == 6;
+    }
+    p.min_depth = c->min_depth;
+    p.max_depth = c->max_depth;
+    p.depth_limit = c->depth_limit;
+    p.tourn = c->tournament_size;
+    p.elitism = c->elitism;
+    p.pop = c->population;
+    p.n_off = n_off;
+    // CPython sample(): setsize = 21 (+ 4 ** ceil(log(3k, 4)) for k > 5)
+    const int k = std::min(c->tournament_size, c->population);
+    int setsize = 21;
+    if (k > 5) setsize += (int)std::pow(4.0, std::ceil(std::log((double)(k * 3)) / std::log(4.0)));
+    if (setsize > MAX_POOL) return gp_fail("tournament too large for the sampling pool");
+    p.setsize = setsize;
+    p.p_cx = c->crossover_rate;
+    p.p_cxmut = c->crossover_rate + c->mutation_rate;
+    p.p_term = (double)p.nt / ((double)p.nt + (double)p.nf);
+
+    uint64_t mk = 0;
+    for (int s = 0; s < SIGP_N_TERMS; ++s)
+        if (term_mask[s]) mk |= bit(s);
+    uint64_t rd_all, comp_all, fl_all;
+    compute_sets(vocab, mk, &rd_all, &comp_all, &fl_all);
+
+    sigp_gp *g = new sigp_gp();
+    g->inst = inst;
+    g->prm = p;
+    g->runs = c->n_runs;
+    g->T = c->T;
+    g->n = sigp_internal_instance_n(inst);
+    g->mask = mk;
+    g->P = c->n_runs * c->population;
+    const size_t P = (size_t)g->P, O = (size_t)c->n_runs * n_off, NL = P * MAXL, OL = O * MAXL;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 8 + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_mt = take((size_t)g->runs * MT_WORDS * 4), o_ptok = take(NL * 2), o_otok = take(OL * 2),
+                 o_grow = take(OL * 2), o_plen = take(P * 4), o_olen = take(O * 4), o_rec = take(O * sizeof(Recipe)),
+                 o_size = take(NL * 2), o_pos = take(NL * 2), o_dep = take(NL), o_hei = take(NL), o_pmax = take(NL),
+                 o_smax = take(NL), o_nint = take(P * 4), o_rops = take(NL * 4), o_rargs = take(NL * 8),
+                 o_ops = take(NL * 4), o_args = take(NL * 8), o_off = take(P * 4), o_len = take(P * 4),
+                 o_rd = take(P * 8), o_fl = take(P * 8), o_cost = take(P * 4), o_ord = take(P * 4),
+                 o_fit = take(P * 8), o_pairs = take(P * (size_t)g->T * 8),
+                 o_scen = take((size_t)g->runs * g->T * g->n * 8), o_gen = take(4);
+    if (hipMalloc((void **)&g->dbuf, off) != hipSuccess) {
+        delete g;
+        return sigp_internal_fail(SIGP_OOM, "GP allocation failed");
+    }
+    char *b = g->dbuf;
+    g->mt = (uint32_t *)(b + o_mt);
+    g->ptok = (int16_t *)(b + o_ptok);
+    g->otok = (int16_t *)(b + o_otok);
+    g->grow = (int16_t *)(b + o_grow);
+    g->plen = (int32_t *)(b + o_plen);
+    g->olen = (int32_t *)(b + o_olen);
+    g->rec = (Recipe *)(b + o_rec);
+    TreeSet &S = g->pset;
+    S.tok = g->ptok;
+    S.len = g->plen;
+    S.count = g->P;
+    S.size = (int16_t *)(b + o_size);
+    S.pos = (int16_t *)(b + o_pos);
+    S.depth = (int8_t *)(b + o_dep);
+    S.height = (int8_t *)(b + o_hei);
+    S.pmax = (int8_t *)(b + o_pmax);
+    S.smax = (int8_t *)(b + o_smax);
+    S.nint = (int32_t *)(b + o_nint);
+    S.raw_ops = (int32_t *)(b + o_rops);
+    S.raw_args = (int32_t *)(b + o_rargs);
+    S.ops = (int32_t *)(b + o_ops);
+    S.args = (int32_t *)(b + o_args);
+    S.off = (int32_t *)(b + o_off);
+    S.plen = (int32_t *)(b + o_len);
+    S.rd = (uint64_t *)(b + o_rd);
+    S.flags = (uint64_t *)(b + o_fl);
+    S.cost = (float *)(b + o_cost);
+    g->order = (int32_t *)(b + o_ord);
+    g->fit = (double *)(b + o_fit);
+    g->pairs = (double *)(b + o_pairs);
+    g->scen = (double *)(b + o_scen);
+    g->gen = (int *)(b + o_gen);
+
+    std::vector<uint32_t> st((size_t)g->runs * MT_WORDS);
+    for (int r = 0; r < g->runs; ++r) mt_seed(seeds[r], st.data() + (size_t)r * MT_WORDS);
+    sigp_status status = SIGP_OK;
+    if (hipMemcpy(g->mt, st.data(), st.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        status = sigp_internal_fail(SIGP_DEVICE_ERROR, "GP set-up failed");
+    if (status == SIGP_OK) {
+        sigp_device_rules R;
+        R.ops = S.ops;
+        R.args = S.args;
+        R.off = S.off;
+        R.len = S.plen;
+        R.rd = S.rd;
+        R.flags = S.flags;
+        R.order = g->order;
+        R.P = g->P;
+        R.KL = std::max(1, __builtin_popcountll(rd_all));
+        R.SL = std::max(1, std::max(c->max_depth, c->depth_limit));
+        R.has_rmod = rmod;
+        R.has_rare = (rd_all & RARE_SLOTS) != 0;
+        status = sigp_internal_batch_device(inst, R, g->T, c->population, c->launch, &g->bt);
+    }
+    if (status != SIGP_OK) {
+        sigp_gp_destroy(g);
+        return status;
+    }
+    *out = g;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_destroy(sigp_gp *g) {
+    if (!g) return SIGP_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->bt) sigp_batch_destroy(g->bt);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    if (g->dbuf) (void)hipFree(g->dbuf);
+    delete g;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_init(sigp_gp *g) {
+    if (!g) return gp_fail("gp is NULL");
+    init_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->mt, g->ptok, g->plen);
+    launch_rebuild(g);
+    GP_TRY(hipGetLastError());
+    GP_TRY(hipStreamSynchronize(g->stream));
+    g->ready = true;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_evaluate(sigp_gp *g, const double *sampled, int64_t n_rows, double *fitness) {
+    if (!g || !sampled) return gp_fail("NULL argument");
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    if (n_rows != (int64_t)g->runs * g->T)
+        return gp_fail("need n_runs x T = " + std::to_string((long long)g->runs * g->T) + " scenario rows");
+    GP_TRY(hipMemcpyAsync(g->scen, sampled, (size_t)n_rows * g->n * 8, hipMemcpyHostToDevice, g->stream));
+    sigp_status st = sigp_batch_run(g->bt, g->scen, n_rows, g->pairs, g->fit, g->stream);
+    if (st != SIGP_OK) return st;
+    if (fitness) GP_TRY(hipMemcpyAsync(fitness, g->fit, (size_t)g->P * 8, hipMemcpyDeviceToHost, g->stream));
+    GP_TRY(hipStreamSynchronize(g->stream));
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_step(sigp_gp *g, const double *fitness) {
+    if (!g) return gp_fail("gp is NULL");
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    if (fitness) GP_TRY(hipMemcpyAsync(g->fit, fitness, (size_t)g->P * 8, hipMemcpyHostToDevice, g->stream));
+    launch_step(g);
+    GP_TRY(hipGetLastError());
+    GP_TRY(hipStreamSynchronize(g->stream));
+    return SIGP_OK;
+}
+
+// One generation of the loop on the stream: this generation's rows, the
+// evaluation and its record, and (unless last) the step to the next.
+static sigp_status launch_generation(sigp_gp *g, const double *all, double *hist, bool step) {
+    const long long per_gen = (long long)g->runs * g->T * g->n;
+    rows_kernel<<<(unsigned)std::min<long long>((per_gen + 255) / 256, 1024), 256, 0, g->stream>>>(all, g->gen,
+                                                                                                   per_gen, g->scen);
+    sigp_status st = sigp_batch_run(g->bt, g->scen, (int64_t)g->runs * g->T, g->pairs, g->fit, g->stream);
+    if (st != SIGP_OK) return st;
+    record_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->fit, g->gen, g->P, hist);
+    if (step) launch_step(g);
+    tick_kernel<<<1, 64, 0, g->stream>>>(g->gen);
+    GP_TRY(hipGetLastError());
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_rows, int32_t generations,
+                                   int32_t use_graph, double *fitness_history, double *elapsed_ms) {
+    if (!g || !sampled) return gp_fail("NULL argument");
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    if (generations < 1) return gp_fail("generations must be >= 1");
+    const long long rows_per_gen = (long long)g->runs * g->T;
+    if (n_rows != rows_per_gen * generations)
+        return gp_fail("need generations x n_runs x T = " + std::to_string(rows_per_gen * generations) + " rows");
+    const size_t sbytes = (size_t)n_rows * g->n * 8, hbytes = (size_t)generations * g->P * 8;
+    double *all = nullptr, *hist = nullptr;
+    GP_TRY(hipMalloc((void **)&all, sbytes));
+    if (hipMalloc((void **)&hist, hbytes) != hipSuccess) {
+        (void)hipFree(all);
+        return sigp_internal_fail(SIGP_OOM, "history allocation failed");
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    sigp_status st = SIGP_OK;
+    auto check = [&](hipError_t e, const char *what) {
+        if (e != hipSuccess && st == SIGP_OK)
+            st = sigp_internal_fail(SIGP_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
+        return st == SIGP_OK;
+    };
+    if (check(hipMemcpyAsync(all, sampled, sbytes, hipMemcpyHostToDevice, g->stream), "upload") &&
+        check(hipEventCreate(&e0), "event") && check(hipEventCreate(&e1), "event")) {
+        set_kernel<<<1, 64, 0, g->stream>>>(g->gen, 0);
+        if (use_graph && generations > 1) {
+            // capture one stepping generation; the counter it reads advances on the device
+            if (check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "begin capture")) {
+                const sigp_status s2 = launch_generation(g, all, hist, true);
+                const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+                if (s2 != SIGP_OK && st == SIGP_OK) st = s2;
+                if (check(ec, "end capture")) check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+            }
+        }
+        if (st == SIGP_OK) check(hipEventRecord(e0, g->stream), "event");
+        for (int gi = 0; st == SIGP_OK && gi < generations; ++gi) {
+            const bool step = gi + 1 < generations;
+            if (exec && step) check(hipGraphLaunch(exec, g->stream), "graph launch");
+            else {
+                const sigp_status s2 = launch_generation(g, all, hist, step);
+                if (s2 != SIGP_OK) st = s2;
+            }
+        }
+        if (st == SIGP_OK) check(hipEventRecord(e1, g->stream), "event");
+        if (st == SIGP_OK && check(hipStreamSynchronize(g->stream), "run") && elapsed_ms) {
+            float ms = 0.0f;
+            if (check(hipEventElapsedTime(&ms, e0, e1), "timing")) *elapsed_ms = ms;
+        }
+        if (st == SIGP_OK && fitness_history)
+            check(hipMemcpy(fitness_history, hist, hbytes, hipMemcpyDeviceToHost), "history download");
+    }
+    (void)hipStreamSynchronize(g->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(all);
+    (void)hipFree(hist);
+    return st;
+}
+
+extern "C" sigp_status sigp_gp_population(const sigp_gp *g, int16_t *tokens, int32_t *lengths) {
+    if (!g || !tokens || !lengths) return gp_fail("NULL argument");
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    GP_TRY(hipStreamSynchronize(g->stream));
+    GP_TRY(hipMemcpy(tokens, g->ptok, (size_t)g->P * MAXL * 2, hipMemcpyDeviceToHost));
+    GP_TRY(hipMemcpy(lengths, g->plen, (size_t)g->P * 4, hipMemcpyDeviceToHost));
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_packed(const sigp_gp *g, int32_t *ops, int32_t *args, int32_t *lengths) {
+    if (!g || !ops || !args || !lengths) return gp_fail("NULL argument");
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    GP_TRY(hipStreamSynchronize(g->stream));
+    GP_TRY(hipMemcpy(ops, g->pset.raw_ops, (size_t)g->P * MAXL * 4, hipMemcpyDeviceToHost));
+    GP_TRY(hipMemcpy(args, g->pset.raw_args, (size_t)g->P * MAXL * 8, hipMemcpyDeviceToHost));
+    GP_TRY(hipMemcpy(lengths, g->plen, (size_t)g->P * 4, hipMemcpyDeviceToHost));
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_rng_state(const sigp_gp *g, int32_t run, uint32_t *state) {
+    if (!g || !state) return gp_fail("NULL argument");
+    if (run < 0 || run >= g->runs) return gp_fail("run out of range");
+    GP_TRY(hipStreamSynchronize(g->stream));
+    GP_TRY(hipMemcpy(state, g->mt + (size_t)run * MT_WORDS, MT_WORDS * 4, hipMemcpyDeviceToHost));
+    return SIGP_OK;
+}

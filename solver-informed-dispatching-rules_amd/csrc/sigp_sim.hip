@@ -35,6 +35,7 @@
 #include "sigp_libm.h"
 #include "sigp_terms.h"
 #include "sigp_fused.h"
+#include "sigp_internal.h"
 
 using namespace sigp;
 
@@ -992,59 +993,6 @@ extern "C" sigp_status sigp_zstats(const sigp_instance_desc *desc, double *zp_ou
     return sigp_instance_destroy(I);
 }
 
-// Slots computed by their own per-candidate routines rather than the fused
-// scan families (decide's RARE variant).
-static constexpr uint64_t RARE_SLOTS = B(44) | B(49) | B(50) | B(53) | B(54) | B(55) | B(56) |
-                                       B(57) | B(59) | B(60);
-
-// Rule validation and the per-rule compute set.
-// A slot's value is live in the reference when (engine/sim_core.hpp:88-197):
-// slots 0-9 always; 11/14/30 when their own or their Z partner's mask bit is
-// set (33/32/34); any other slot when its mask bit is set.  A leaf whose slot
-// is not live reads 0.0 and is rewritten to the constant 0.
-static bool slot_live(int s, uint64_t mk) {
-    if (s <= 9) return true;
-    if (s == 11) return (mk & (B(11) | B(33))) != 0;
-    if (s == 14) return (mk & (B(14) | B(32))) != 0;
-    if (s == 30) return (mk & (B(30) | B(34))) != 0;
-    return (mk & B(s)) != 0;
-}
-
-// Slots whose value the rule reads (rd_out), the same closed under the Z
-// slots' sources (comp_out), and the fused computations that produce them.
-static void compute_sets(uint64_t used, uint64_t mk, uint64_t *rd_out, uint64_t *comp_out,
-                         uint64_t *fl_out) {
-    uint64_t c = 0;
-    for (int s = 0; s < SIGP_N_TERMS; ++s)
-        if ((used & B(s)) && slot_live(s, mk)) c |= B(s);
-    *rd_out = c;
-    if (c & B(32)) c |= B(14);
-    if (c & B(33)) c |= B(11);
-    if (c & B(34)) c |= B(30);
-    uint64_t f = 0;
-    if (c & (B(10) | B(28) | B(29) | B(45))) f |= F_NS;
-    if (c & (B(11) | B(43))) f |= F_REGRET;
-    if (c & (B(12) | B(45) | B(46))) f |= F_NSROB;
-    if (c & B(13)) f |= F_MAXN;
-    if (c & B(14)) f |= F_REACH2;
-    if (c & B(15)) f |= F_NSTW;
-    if (c & (B(17) | B(18) | B(19) | B(20) | B(28) | B(29))) f |= F_STRUCT;
-    if (c & (B(21) | B(22) | B(23))) {
-        const uint64_t k = (c & B(23)) ? 5 : ((c & B(22)) ? 3 : 1);
-        f |= F_REACHK | (k << 40);
-    }
-    if (c & B(30)) f |= F_ATRISK | F_CNOW;
-    if (c & (B(41) | B(43))) f |= F_R2ROB;
-    if (c & (B(47) | B(48))) {
-        const uint64_t k = (c & B(48)) ? 5 : 3;
-        f |= F_REACHKR | (k << 43);
-    }
-    if (c & B(51)) f |= F_FSCORE;
-    if (c & B(52)) f |= F_FNOWROB;
-    *comp_out = c;
-    *fl_out = f;
-}
-
 // Validated rules, rewritten for the device: dead leaves become the constant
 // 0, live leaves index the compact value store; per rule the read set, the
 // fused computations and a relative cost estimate.
@@ -1128,22 +1076,7 @@ static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, 
         pr.rd[p] = r;
         pr.any_rd |= r;
         pr.flags[p] = f;
-        // Relative cost per candidate from the scans the flags select (the
-        // per-neighbour weights were measured on MI355X): the launch order
-        // puts expensive rules first so that they do not form its tail.
-        float w = 1.0f + 0.05f * (float)L;
-        if (f & (F_NS | F_MAXN | F_REACH2)) w += 1.0f;
-        if (f & F_REACH2) w += 1.0f;
-        if (f & (F_NSROB | F_R2ROB | F_REACHKR)) w += 1.0f;
-        if (f & F_R2ROB) w += 1.0f;
-        if (f & F_REGRET) w += 0.5f;
-        if (f & F_STRUCT) w += 2.0f;
-        if (f & F_NSTW) w += 1.0f;
-        if (f & F_ATRISK) w += 8.0f;
-        if (f & F_REACHK) w += 1.5f * (float)((f >> 40) & 7);
-        if (f & F_REACHKR) w += 1.5f * (float)((f >> 43) & 7);
-        if (c & RARE_SLOTS) w += 4.0f;
-        pr.cost[p] = w;
+        pr.cost[p] = rule_cost((int)L, f, c);
     }
     return SIGP_OK;
 }
@@ -1368,12 +1301,14 @@ static int fit_waves(const sigp_batch *bt, int w) {
 }
 
 // A size class's tuned mapping, else the batch's; AUTO: candidate lanes below
-// 16 scenarios per rule (scenario lanes would be mostly idle), scenario lanes
-// otherwise.
+// 16 scenarios per rule (scenario lanes would be mostly idle) or when
+// scenario lanes would make fewer than 512 workgroups (too few to fill 256
+// CUs, e.g. one rule scored on a held-out set), scenario lanes otherwise.
 static int resolved_mapping(const sigp_batch *bt, const LaunchUnit &u) {
     if (u.mapping != SIGP_MAP_AUTO) return u.mapping;
     if (bt->mapping != SIGP_MAP_AUTO) return bt->mapping;
-    return bt->T < 16 ? SIGP_MAP_CANDIDATES : SIGP_MAP_SCENARIOS;
+    const long long scen_items = u.n_pairs * ((bt->T + 63) / 64);
+    return (bt->T < 16 || scen_items < 512) ? SIGP_MAP_CANDIDATES : SIGP_MAP_SCENARIOS;
 }
 
 // Wavefronts per workgroup: the tuned or configured count, else 4 for
@@ -1631,6 +1566,30 @@ extern "C" sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t 
     return st;
 }
 
+// Phenotype launch on `stream`: one workgroup per rule, as many wavefronts
+// as LDS allows beside the kernel's 1024 static priorities.
+static void launch_pheno(const sigp_batch *bt, const PhenoSpec &Ph, int32_t *outd, hipStream_t stream) {
+    int w = 4;
+    while (w > 1 && 1024 * sizeof(double) + (size_t)w * lds_per_wave(bt) > LDS_LIMIT) w >>= 1;
+    const unsigned lds = (unsigned)(w * lds_per_wave(bt));
+    const int P = bt->P;
+    const int nwb = nw_bucket(bt->insts[0]->n);
+#define PHENO(NWV)                                                                                   \
+    do {                                                                                             \
+        if (bt->has_rmod) {                                                                          \
+            if (bt->has_rare) pheno_kernel<NWV, true, true><<<P, 64 * w, lds, stream>>>(bt->slots, bt->rs, Ph, outd); \
+            else pheno_kernel<NWV, true, false><<<P, 64 * w, lds, stream>>>(bt->slots, bt->rs, Ph, outd); \
+        } else {                                                                                     \
+            if (bt->has_rare) pheno_kernel<NWV, false, true><<<P, 64 * w, lds, stream>>>(bt->slots, bt->rs, Ph, outd); \
+            else pheno_kernel<NWV, false, false><<<P, 64 * w, lds, stream>>>(bt->slots, bt->rs, Ph, outd); \
+        }                                                                                            \
+    } while (0)
+    if (nwb == 4) PHENO(4);
+    else if (nwb == 9) PHENO(9);
+    else PHENO(16);
+#undef PHENO
+}
+
 extern "C" sigp_status sigp_phenotype(sigp_instance *inst, const sigp_rules *rules, const int64_t *masks,
                                       int32_t n_masks, int32_t mask_group, const sigp_situations *sit,
                                       int32_t *out_ranks) {
@@ -1681,25 +1640,8 @@ extern "C" sigp_status sigp_phenotype(sigp_instance *inst, const sigp_rules *rul
         Ph.cand = (const int *)(d + o_cand);
         Ph.ct = (const double *)(d + o_ct);
         Ph.vis = (const uint32_t *)(d + o_vis);
-        int w = 4;   // the kernel's static LDS holds 1024 priorities
-        while (w > 1 && 1024 * sizeof(double) + (size_t)w * lds_per_wave(bt) > LDS_LIMIT) w >>= 1;
-        const unsigned lds = (unsigned)(w * lds_per_wave(bt));
         int32_t *outd = (int32_t *)(d + o_out);
-        const int nwb = nw_bucket(n);
-#define PHENO(NWV)                                                                                   \
-    do {                                                                                             \
-        if (bt->has_rmod) {                                                                          \
-            if (bt->has_rare) pheno_kernel<NWV, true, true><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
-            else pheno_kernel<NWV, true, false><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
-        } else {                                                                                     \
-            if (bt->has_rare) pheno_kernel<NWV, false, true><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
-            else pheno_kernel<NWV, false, false><<<pr.P, 64 * w, lds>>>(bt->slots, bt->rs, Ph, outd); \
-        }                                                                                            \
-    } while (0)
-        if (nwb == 4) PHENO(4);
-        else if (nwb == 9) PHENO(9);
-        else PHENO(16);
-#undef PHENO
+        launch_pheno(bt, Ph, outd, nullptr);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipDeviceSynchronize();
         if (e == hipSuccess) e = hipMemcpy(out_ranks, outd, (size_t)pr.P * S * C * 4, hipMemcpyDeviceToHost);
@@ -1731,5 +1673,83 @@ extern "C" sigp_status sigp_device_math(int32_t which, const double *x, const do
     (void)hipFree(dx);
     (void)hipFree(dy);
     (void)hipFree(dout);
+    return SIGP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// interfaces for the library's other translation units
+
+int sigp_internal_instance_n(const sigp_instance *inst) { return inst->n; }
+
+sigp_status sigp_internal_batch_device(sigp_instance *inst, const sigp_device_rules &r, int T, int group_size,
+                                       const sigp_launch_config &cfg, sigp_batch **out) {
+    if (!inst || !out) return fail(SIGP_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    sigp_status st = check_config(&cfg);
+    if (st != SIGP_OK) return st;
+    if (r.P < 1 || T < 1 || group_size < 1) return fail(SIGP_INVALID_ARG, "empty device batch");
+    if (r.KL < 1 || r.KL > SIGP_N_TERMS || r.SL < 1 || r.SL > SIGP_MAX_STACK)
+        return fail(SIGP_INVALID_ARG, "value-store / stack bounds out of range");
+    sigp_batch *bt = new sigp_batch();
+    bt->insts = {inst};
+    bt->P = r.P;
+    bt->T = T;
+    bt->group_size = group_size;
+    bt->waves_per_item = cfg.waves_per_item;
+    bt->mapping = cfg.mapping;
+    bt->rows_needed = (int64_t)((r.P - 1) / group_size + 1) * T;
+    bt->n_out = (long long)r.P * T;
+    bt->has_rmod = r.has_rmod;
+    bt->has_rare = r.has_rare;
+    InstSlot slot;
+    slot.dev = inst->dev;
+    slot.tabs = inst->tabs;
+    slot.samp_off = 0;
+    slot.out_off = 0;
+    if (hipMalloc(&bt->dbuf, sizeof(InstSlot)) != hipSuccess) {
+        delete bt;
+        return fail(SIGP_OOM, "batch allocation failed");
+    }
+    if (hipMemcpy(bt->dbuf, &slot, sizeof(InstSlot), hipMemcpyHostToDevice) != hipSuccess) {
+        sigp_batch_destroy(bt);
+        return fail(SIGP_DEVICE_ERROR, "batch upload failed");
+    }
+    bt->slots = (cslot)bt->dbuf;
+    bt->rs.ops = (cip)r.ops;
+    bt->rs.args = (cip)r.args;
+    bt->rs.off = (cip)r.off;
+    bt->rs.len = (cip)r.len;
+    bt->rs.rd = (cup)r.rd;
+    bt->rs.flags = (cup)r.flags;
+    bt->rs.order = nullptr;
+    bt->rs.P = r.P;
+    bt->rs.KL = r.KL;
+    bt->rs.SL = r.SL;
+    LaunchUnit &u = bt->units[0];
+    u.nwb = nw_bucket(inst->n);
+    u.max_n = inst->n;
+    u.n_pairs = r.P;
+    u.order = (cip)r.order;
+    u.tune_order = (cip)r.order;
+    u.tune_pairs = r.P;
+    bt->n_units = 1;
+    *out = bt;
+    return SIGP_OK;
+}
+
+sigp_status sigp_internal_phenotype_device(const sigp_batch *bt, const sigp_device_situations &sit,
+                                           int32_t *out_dev, hipStream_t stream) {
+    if ((long long)sit.n_sit * sit.n_cand > 1024 || sit.n_sit < 1 || sit.n_cand < 1)
+        return fail(SIGP_INVALID_ARG, "need 1 <= n_sit * n_cand <= 1024");
+    PhenoSpec Ph;
+    Ph.n_sit = sit.n_sit;
+    Ph.n_cand = sit.n_cand;
+    Ph.cur = sit.cur;
+    Ph.rem = sit.rem;
+    Ph.cand = sit.cand;
+    Ph.ct = sit.ct;
+    Ph.vis = sit.vis;
+    launch_pheno(bt, Ph, out_dev, stream);
+    HIP_TRY(hipGetLastError());
     return SIGP_OK;
 }

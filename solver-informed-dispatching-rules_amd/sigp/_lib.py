@@ -9,7 +9,7 @@ import os
 from typing import Optional
 
 __all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "Situations",
-           "EXPORTED",
+           "GpConfig", "EXPORTED", "GP_MAX_NODES", "GP_TOK_FUNC", "GP_TOK_CONST",
            "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -22,7 +22,8 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
             "sigp_evaluate_multi", "sigp_phenotype", "sigp_mlp_create", "sigp_mlp_destroy",
             "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
-            "sigp_mlp_predict"]
+            "sigp_mlp_predict", "sigp_gp_create", "sigp_gp_destroy", "sigp_gp_init", "sigp_gp_evaluate",
+            "sigp_gp_step", "sigp_gp_run", "sigp_gp_population", "sigp_gp_packed", "sigp_gp_rng_state"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
            4: "SIGP_NO_DEVICE"}
@@ -51,6 +52,18 @@ class LaunchConfig(C.Structure):
 class Situations(C.Structure):
     _fields_ = [("n_sit", C.c_int32), ("n_cand", C.c_int32), ("cur", C.c_void_p), ("ct", C.c_void_p),
                 ("remaining", C.c_void_p), ("visited", C.c_void_p), ("cand", C.c_void_p)]
+
+
+GP_MAX_NODES, GP_TOK_FUNC, GP_TOK_CONST = 512, 0x4000, 0x2000
+
+
+class GpConfig(C.Structure):
+    _fields_ = [("n_runs", C.c_int32), ("population", C.c_int32), ("offspring", C.c_int32),
+                ("n_terminals", C.c_int32), ("terminal_slots", C.c_void_p), ("n_functions", C.c_int32),
+                ("function_ops", C.c_void_p), ("min_depth", C.c_int32), ("max_depth", C.c_int32),
+                ("depth_limit", C.c_int32), ("tournament_size", C.c_int32), ("elitism", C.c_int32),
+                ("crossover_rate", C.c_double), ("mutation_rate", C.c_double), ("T", C.c_int32),
+                ("launch", LaunchConfig)]
 
 
 class Rules(C.Structure):
@@ -103,6 +116,15 @@ def lib() -> C.CDLL:
         "sigp_mlp_gradients": ([vp, vp, vp, i32, vp, vp], C.c_int),
         "sigp_mlp_train": ([vp, vp, vp, i64, i32, i32, C.c_float, C.c_uint64, vp], C.c_int),
         "sigp_mlp_predict": ([vp, vp, i64, vp], C.c_int),
+        "sigp_gp_create": ([vp, vp, vp, vp, vp], C.c_int),
+        "sigp_gp_destroy": ([vp], C.c_int),
+        "sigp_gp_init": ([vp], C.c_int),
+        "sigp_gp_evaluate": ([vp, vp, i64, vp], C.c_int),
+        "sigp_gp_step": ([vp, vp], C.c_int),
+        "sigp_gp_run": ([vp, vp, i64, i32, i32, vp, vp], C.c_int),
+        "sigp_gp_population": ([vp, vp, vp], C.c_int),
+        "sigp_gp_packed": ([vp, vp, vp, vp], C.c_int),
+        "sigp_gp_rng_state": ([vp, i32, vp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(L, name)
