@@ -111,6 +111,83 @@ def describe(kind, wl):
                                                                       wl.group_size))
 
 
+def bench_gp_loop(args):
+    """The whole GP loop on the device: clustered instance n=100 m=3, one run
+    of 1,024 rules over NS-11 + EXTRA-18 (ramped half-and-half, depth limit
+    8, tournament 7, crossover 0.8 / mutation 0.15, 10 elites), 50 training
+    scenarios per generation (Gamma 4, seeds run*10000 + generation, as
+    engine/gp_engine.cu:69), 50 generations, one generation per HIP-graph
+    replay.  A step is one generation: evaluation plus breeding."""
+    import torch  # noqa: F401  (one HIP runtime for torch and libsigp)
+    from sigp import gp as G
+    from sigp.api import FactoryContext, device_info
+    from sigp.gp_device import DeviceGP
+    from sigp.rules import EXTRA, FUNCTIONS, NS_TERMINALS, pack_trees, term_mask_from
+    from sigp.scenarios import gen_durations
+    from sigp.workloads import TEST_SEED, clustered_gp_instance
+
+    gens, T, pop, runs = args.generations, args.scenarios or 50, args.rules or 1024, args.runs
+    inst = clustered_gp_instance()
+    vocab = list(NS_TERMINALS) + list(EXTRA)
+    mask = term_mask_from(vocab)
+    tc = G.TreeConfig(vocab, list(FUNCTIONS))
+    ec = G.EvolutionConfig(population_size=pop)
+    t0 = time.perf_counter()
+    rows = np.concatenate([gen_durations(inst.durations, T, r * 10000 + g, mode="gamma", shape=4.0)
+                           for g in range(gens) for r in range(runs)])
+    log("scenarios for %d generations built in %.1fs" % (gens, time.perf_counter() - t0))
+    name, n_cu, _ = device_info()
+    log("device", name, n_cu, "CUs")
+    ctx = FactoryContext(inst)
+    seeds = list(range(runs))
+    launch = {}
+    if args.waves:
+        launch = dict(waves_per_item=args.waves, mapping=1 if args.mapping == "scenarios" else 2)
+    # warm-up: code objects, graph instantiation
+    gp = DeviceGP(ctx, tc, ec, seeds=seeds, T=T, term_mask=mask, **launch)
+    gp.init()
+    gp.run(rows[:2 * runs * T], generations=2, graph=True)
+    gp.close()
+    gp = DeviceGP(ctx, tc, ec, seeds=seeds, T=T, term_mask=mask, **launch)
+    gp.init()
+    wall0 = time.perf_counter()
+    hist, dev_ms = gp.run(rows, generations=gens, graph=not args.no_graph)
+    wall = time.perf_counter() - wall0
+    log("device loop: %d generations in %.1f ms device time, %.3f s wall" % (gens, dev_ms, wall))
+    # the reference's read-out: best of the final population on one more
+    # training draw, then its mean over 500 held-out scenarios
+    final = gp.population()
+    best_rules = []
+    for r in range(runs):
+        fs = gen_durations(inst.durations, T, r * 10000 + gens, mode="gamma", shape=4.0)
+        _, fm = ctx.evaluate_means(pack_trees(final[r]), fs, T, mask)
+        best_rules.append(final[r][int(np.argmax(fm))])
+    test = gen_durations(inst.durations, 500, TEST_SEED, mode="gamma", shape=4.0)
+    _, held = ctx.evaluate_means(pack_trees(best_rules), test, 500, mask)
+    log("held-out 500-scenario mean reward of each run's best rule:", np.round(held, 3))
+    sims = gens * runs * pop * T
+    ms_gen = dev_ms / gens
+    out = {
+        "metric": "rule x scenario sims/sec (whole GP loop: evaluation + breeding on the device)",
+        "value": round(sims / (dev_ms / 1000.0), 1), "unit": "sims/s", "n_gpus": 1, "steps": gens,
+        "warmup": 2, "ms_per_step": round(ms_gen, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "GP loop on the device: clustered instance n=100 m=3 (windows U[30,120]), "
+                               "%d run(s) x %d rules (NS-11 + EXTRA-18, depth <= 8), %d Gamma(4) training "
+                               "scenarios per generation, %d generations, %s"
+                               % (runs, pop, T, gens, "one HIP graph per generation" if not args.no_graph
+                                  else "stream launches"),
+                   "rules": pop * runs, "scenarios": T, "generations": gens, "n": inst.n, "m": inst.m,
+                   "parallelism": "1 GPU; evaluation lanes: %s" % (
+                       "automatic" if not args.waves else "%s, %d wavefronts" % (args.mapping, args.waves))},
+        "wall_s": round(wall, 3),
+        "held_out_reward": [round(float(v), 3) for v in held],
+        "best_train_fitness_per_generation": [round(float(v), 3) for v in hist[:, 0].max(axis=1)],
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,14 +195,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
     ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
-    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large"],
+    ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large",
+                                           "gp-loop"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
                          "gp-reference: one launch of the reference's GP loop (30 seeds x 1,024 "
                          "rules x 1 scenario); batched: 40 uneven instances (n 48-288) x 256 "
                          "rules x 500 scenarios in one multi-instance batch; large: n=288 m=4, "
-                         "4,096 rules x 2,000 scenarios")
+                         "4,096 rules x 2,000 scenarios; gp-loop: the whole GP loop on the "
+                         "device (1,024 rules x 50 scenarios x 50 generations, breeding included)")
+    ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
+    ap.add_argument("--runs", type=int, default=1, help="gp-loop: independent runs side by side")
+    ap.add_argument("--no-graph", action="store_true", help="gp-loop: stream launches, no HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--waves", type=int, default=0,
@@ -140,6 +222,8 @@ def main():
               "(each GPU would evaluate its own share of rules); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "gp-loop":
+        return bench_gp_loop(args)
     import torch
     from sigp.api import FactoryContext, batch_multi, device_info
     from sigp.workloads import (batched_uneven, gp_generation, gp_reference_generation, large,

@@ -146,7 +146,7 @@ sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst, con
  *     take the first maximum together.  For few scenarios per rule, such as
  *     the reference's GP loop, which evaluates one per generation
  *     (engine/gp.hpp:32).
- *   SIGP_MAP_AUTO: candidate lanes when T < 16 or when scenario lanes would
+ *   SIGP_MAP_AUTO: candidate lanes when T < 64 or when scenario lanes would
  *     make fewer than 512 workgroups, scenario lanes otherwise.
  * Each wavefront keeps the candidates' terminal values and the postfix stack
  * in LDS, (distinct slots read + deepest stack - 1) * 512 bytes, and a
@@ -266,7 +266,7 @@ typedef struct sigp_gp sigp_gp;
 /* TreeConfig + EvolutionConfig (engine/gp.hpp:13-34). */
 typedef struct {
     int32_t n_runs;                  /* independent runs evolved side by side (the reference's seeds) */
-    int32_t population;              /* EvolutionConfig::population_size, <= 6144 */
+    int32_t population;              /* EvolutionConfig::population_size, <= 4096 */
     int32_t offspring;               /* offspring per run and step: 0 = population */
     int32_t n_terminals;             /* TreeConfig::terminals, <= 128 */
     const int32_t *terminal_slots;   /* slot of each terminal (0..60), -1 = "ERC" */

@@ -55,7 +55,7 @@ constexpr int MAX_TERMS = 128;
 constexpr int MAX_FUNCS = 8;
 constexpr int MAX_POOL = 512;         // sample()'s pool method: population <= setsize
 constexpr int MAX_TOURN = 64;
-constexpr int MAX_POP = 6144;         // the breeding wavefront keeps the fitness in LDS
+constexpr int MAX_POP = 4096;         // the breeding wavefront keeps fitness and tree words in LDS
 constexpr int TOK_FUNC = SIGP_GP_TOK_FUNC;
 constexpr int TOK_CONST = SIGP_GP_TOK_CONST;
 constexpr int MT_WORDS = 625;         // 624 state words + position
@@ -74,22 +74,36 @@ struct GpParams {
 };
 
 enum { R_COPY = 0, R_XOVER = 1, R_MUT = 2 };
-// Offspring o of a run: parent a (within the run), and for crossover the
-// point pa in a replaced by b's subtree at pb; for mutation by the glen
-// tokens grown into the offspring's grow slot.
-struct Recipe {
-    int kind, a, pa, b, pb, glen;
+// Offspring o of a run: parent a (within the run); for crossover the subtree
+// of sa nodes at a's point pa is replaced by b's subtree of sb nodes at pb,
+// for mutation by the sb tokens grown into the offspring's grow slot.  L is
+// the offspring's length.
+struct __align__(16) Recipe {
+    int kind, a, pa, sa, b, pb, sb, L;
 };
 
+// Breeding tables of the parents.  Per tree one word: length | internal
+// nodes << 16.  Per tree MAXL point words, the internal nodes first, then
+// the leaves, each in pre-order: position << 21 | node word, where the node
+// word holds what a crossover or mutation at that node needs --
+// subtree size (bits 0-8), depth (9-12), subtree height (13-16) and
+// 1 + the depth of the tree outside the subtree (17-20; 0 = nothing outside).
+struct Parents {
+    const uint32_t *tw;
+    const uint32_t *pts;
+};
+__device__ __forceinline__ int pw_pos(uint32_t w) { return (int)(w >> 21); }
+__device__ __forceinline__ int pw_size(uint32_t w) { return (int)(w & 511u); }
+__device__ __forceinline__ int pw_depth(uint32_t w) { return (int)((w >> 9) & 15u); }
+__device__ __forceinline__ int pw_height(uint32_t w) { return (int)((w >> 13) & 15u); }
+__device__ __forceinline__ int pw_outside(uint32_t w) { return (int)((w >> 17) & 15u) - 1; }
+
 // A set of trees and what their analysis writes.  Parents need the breeding
-// tables (size .. nint); every evaluated set needs the compiled arrays.
+// tables (tw, pts); every evaluated set needs the compiled arrays.
 struct TreeSet {
     const int16_t *tok;
     const int32_t *len;
-    int count;
-    int16_t *size, *pos;                      // per node; pos: internals, then leaves
-    int8_t *depth, *height, *pmax, *smax;     // per node
-    int32_t *nint;                            // per tree
+    uint32_t *tw, *pts;                       // breeding tables (null: not a parent set)
     int32_t *raw_ops, *raw_args;              // pack_trees format (engine/factory.cuh:20-40)
     int32_t *ops, *args, *off, *plen;         // prepared for the kernels
     uint64_t *rd, *flags;
@@ -97,14 +111,16 @@ struct TreeSet {
 };
 
 // ---------------------------------------------------------------------------
-// CPython's Mersenne Twister (engine/rng.hpp:54-93), state in LDS, driven by
-// the whole wavefront in lockstep.
+// CPython's Mersenne Twister (engine/rng.hpp:54-93), driven by the whole
+// wavefront in lockstep: the 624 state words in LDS, and a window of 64
+// consecutive output words tempered in registers, lane l holding word
+// base + l, so that a draw is one read of a lane.
 
 // Regenerate the 624 words.  Word i reads words i + 1 (old) and i + 397
 // (old for i < 227) or i - 227 (already new): in chunks of 64 ascending
 // indices every such new word lies in an earlier chunk, so the chunks run
 // lane-parallel with loads and stores separated by barriers.
-__device__ void mt_twist(uint32_t *mt) {
+__device__ __noinline__ void mt_twist(uint32_t *mt) {
     const int lane = (int)threadIdx.x;
     for (int base = 0; base < 623; base += 64) {
         const int i = base + lane;
@@ -124,30 +140,42 @@ __device__ void mt_twist(uint32_t *mt) {
     __syncthreads();
 }
 
-struct Rng {
-    uint32_t *mt;
-    int idx;
+__device__ __forceinline__ uint32_t temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
 
-    __device__ uint32_t next() {
-        if (idx >= 624) {
-            mt_twist(mt);
-            idx = 0;
+struct Rng {
+    uint32_t *mt;    // LDS state
+    int idx;         // next state word; 624 = all used, regenerate first
+    int base;        // words base .. base + 63 are tempered in `win`
+    uint32_t win;
+
+    __device__ __forceinline__ uint32_t next() {
+        if (idx >= 624 || idx - base >= 64) {
+            if (idx >= 624) {
+                mt_twist(mt);
+                idx = 0;
+            }
+            base = idx;
+            const int w = base + (int)threadIdx.x;
+            win = w < 624 ? temper(mt[w]) : 0u;
         }
-        uint32_t y = mt[idx++];
-        y ^= y >> 11;
-        y ^= (y << 7) & 0x9d2c5680u;
-        y ^= (y << 15) & 0xefc60000u;
-        y ^= y >> 18;
-        return y;
+        const int k = idx - base;
+        ++idx;
+        return (uint32_t)__builtin_amdgcn_readlane((int)win, k);
     }
     // the 53-bit integer m of random() = m / 2^53
-    __device__ uint64_t bits53() {
+    __device__ __forceinline__ uint64_t bits53() {
         const uint64_t a = next() >> 5, b = next() >> 6;
         return (a << 26) | b;
     }
-    __device__ double random() { return (double)bits53() * (1.0 / 9007199254740992.0); }
+    __device__ __forceinline__ double random() { return (double)bits53() * (1.0 / 9007199254740992.0); }
     // randbelow: rejection on bit_length(n) top bits; 0 for n == 0
-    __device__ uint32_t below(uint32_t n) {
+    __device__ __forceinline__ uint32_t below(uint32_t n) {
         if (n == 0) return 0;
         const int k = 32 - __clz(n);
         uint32_t r = next() >> (32 - k);
@@ -160,7 +188,7 @@ struct Rng {
 // engine/vocab.hpp:101-104) is k = m * 1000 / 2^53 rounded to nearest, ties
 // to even (how printf rounds an exactly representable tie) -- computed
 // exactly in 64-bit integers.
-__device__ int erc_thousandths(uint64_t m) {
+__device__ __forceinline__ int erc_thousandths(uint64_t m) {
     const uint64_t x = m * 1000ull;               // < 2^63
     uint64_t q = x >> 53;
     const uint64_t r = x & ((1ull << 53) - 1);
@@ -170,22 +198,23 @@ __device__ int erc_thousandths(uint64_t m) {
 }
 
 // engine/gp.hpp:37-45
-__device__ int pick_terminal(const GpParams &P, Rng &g) {
+__device__ __forceinline__ int pick_terminal(const GpParams &P, Rng &g) {
     const int t = (int)g.below((uint32_t)P.nt);
     if (P.tslot[t] < 0) return TOK_CONST | erc_thousandths(g.bits53());
     return t;
 }
 
 // engine/gp.hpp:47-64 (generate_full, generate_grow): the recursion as a
-// stack of pending depths, which emits nodes and makes draws in the same
-// pre-order.  Lane 0 stores the tokens; returns the count and the height.
-__device__ int generate(const GpParams &P, Rng &g, int *gst, int depth, bool full, int16_t *out,
-                        int &height) {
-    int sp = 0, cnt = 0;
+// stack of pending depths (lane s holds entry s), which emits nodes and
+// makes draws in the same pre-order.  Lane 0 stores the tokens; returns the
+// count and the height.
+__device__ __forceinline__ int generate(const GpParams &P, Rng &g, int depth, bool full, int16_t *out, int &height) {
+    const int lane = (int)threadIdx.x;
+    int stk = depth, sp = 1, cnt = 0;
     height = 0;
-    gst[sp++] = depth;
     while (sp > 0) {
-        const int d = gst[--sp];
+        --sp;
+        const int d = __builtin_amdgcn_readlane(stk, sp);
         bool term = d <= 0;
         if (!term && !full) term = g.random() < P.p_term;
         int tk;
@@ -194,23 +223,25 @@ __device__ int generate(const GpParams &P, Rng &g, int *gst, int depth, bool ful
             height = max(height, depth - d);
         } else {
             tk = TOK_FUNC | (int)g.below((uint32_t)P.nf);
-            gst[sp++] = d - 1;
-            gst[sp++] = d - 1;
+            if (lane == sp || lane == sp + 1) stk = d - 1;
+            sp += 2;
         }
-        if (threadIdx.x == 0) out[cnt] = (int16_t)tk;
+        if (lane == 0) out[cnt] = (int16_t)tk;
         ++cnt;
     }
     return cnt;
 }
 
-__device__ void load_rng(Rng &g, uint32_t *mt, const uint32_t *state) {
+__device__ __forceinline__ void load_rng(Rng &g, uint32_t *mt, const uint32_t *state) {
     for (int k = (int)threadIdx.x; k < 624; k += 64) mt[k] = state[k];
     __syncthreads();
     g.mt = mt;
     g.idx = (int)state[624];
+    g.base = -1000;   // the window is filled at the first draw
+    g.win = 0;
 }
 
-__device__ void store_rng(const Rng &g, uint32_t *state) {
+__device__ __forceinline__ void store_rng(const Rng &g, uint32_t *state) {
     __syncthreads();
     for (int k = (int)threadIdx.x; k < 624; k += 64) state[k] = g.mt[k];
     if (threadIdx.x == 0) state[624] = (uint32_t)g.idx;
@@ -220,7 +251,6 @@ __device__ void store_rng(const Rng &g, uint32_t *state) {
 // One wavefront per run.
 __global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, int16_t *tok, int32_t *len) {
     __shared__ uint32_t mt[624];
-    __shared__ int gst[2 * MAX_DEPTH + 4];
     const int r = (int)blockIdx.x;
     Rng g;
     load_rng(g, mt, state + (size_t)r * MT_WORDS);
@@ -230,7 +260,7 @@ __global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, i
         const bool full = (i / nd) % 2 == 0;
         int h;
         const size_t gi = (size_t)r * P.pop + i;
-        const int n = generate(P, g, gst, d, full, tok + gi * MAXL, h);
+        const int n = generate(P, g, d, full, tok + gi * MAXL, h);
         if (threadIdx.x == 0) len[gi] = n;
     }
     store_rng(g, state + (size_t)r * MT_WORDS);
@@ -239,75 +269,65 @@ __global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, i
 // ---------------------------------------------------------------------------
 // breeding (engine/gp.hpp:118-194)
 
-struct Parents {
-    const int32_t *len, *nint;
-    const int16_t *size, *pos;
-    const int8_t *depth, *height, *pmax, *smax;
-};
-
 // engine/gp.hpp:120-131: Koza 90/10; the draw of the class happens only
-// when the tree has internal nodes (it always has leaves).
-__device__ int select_point(const Parents &T, size_t gi, Rng &g) {
-    const int L = T.len[gi], ni = T.nint[gi], nl = L - ni;
-    const int16_t *pos = T.pos + gi * MAXL;
-    if (ni > 0 && (nl == 0 || g.random() < 0.9)) return pos[g.below((uint32_t)ni)];
-    if (nl > 0) return pos[ni + (int)g.below((uint32_t)nl)];
-    return pos[g.below((uint32_t)ni)];
-}
-
-// Depth of the nodes of tree gi outside the subtree rooted at p (-1: none).
-__device__ int outside_depth(const Parents &T, size_t gi, int p) {
-    const int L = T.len[gi];
-    const int e = p + T.size[gi * MAXL + p];
-    int d = p > 0 ? T.pmax[gi * MAXL + p - 1] : -1;
-    if (e < L) d = max(d, (int)T.smax[gi * MAXL + e]);
-    return d;
+// when the tree has internal nodes (it always has leaves).  Returns the
+// index into the tree's point list.
+__device__ __forceinline__ int select_point(uint32_t tw, Rng &g) {
+    const int L = (int)(tw & 0xffffu), ni = (int)(tw >> 16), nl = L - ni;
+    if (ni > 0 && (nl == 0 || g.random() < 0.9)) return (int)g.below((uint32_t)ni);
+    if (nl > 0) return ni + (int)g.below((uint32_t)nl);
+    return (int)g.below((uint32_t)ni);
 }
 
 // engine/gp.hpp:157-166 with engine/rng.hpp sample(): k distinct
-// contenders, the first maximum in contender order wins.
-__device__ int tournament(const GpParams &P, Rng &g, const double *fit, int *res, int *pool) {
+// contenders, the first maximum in contender order wins.  Contender i is
+// held by lane i.
+__device__ __forceinline__ int tournament(const GpParams &P, Rng &g, const double *fit, int *pool) {
+    const int lane = (int)threadIdx.x;
     const int n = P.pop, k = min(P.tourn, n);
+    int c = 0;
     if (n <= P.setsize) {
-        for (int j = (int)threadIdx.x; j < n; j += 64) pool[j] = j;
+        for (int j = lane; j < n; j += 64) pool[j] = j;
         __syncthreads();
         for (int i = 0; i < k; ++i) {
             const int j = (int)g.below((uint32_t)(n - i));
             const int pj = pool[j];
             const int last = pool[n - i - 1];
             __syncthreads();
-            res[i] = pj;
-            pool[j] = last;
+            if (lane == i) c = pj;
+            if (lane == 0) pool[j] = last;
             __syncthreads();
         }
     } else {
         for (int i = 0; i < k; ++i) {
             int j = (int)g.below((uint32_t)n);
-            for (;;) {
-                bool seen = false;
-                for (int q = 0; q < i; ++q) seen |= res[q] == j;
-                if (!seen) break;
-                j = (int)g.below((uint32_t)n);
-            }
-            res[i] = j;
+            while (__ballot(lane < i && c == j) != 0) j = (int)g.below((uint32_t)n);
+            if (lane == i) c = j;
         }
     }
-    int best = res[0];
-    for (int i = 1; i < k; ++i)
-        if (fit[res[i]] > fit[best]) best = res[i];
-    return best;
+    // first maximum in contender order: largest fitness, then lowest lane
+    double f = lane < k ? fit[c] : 0.0;
+    int who = lane < k ? lane : 64;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double of = __shfl_xor(f, off);
+        const int ow = __shfl_xor(who, off);
+        if (ow < 64 && (who == 64 || of > f || (of == f && ow < who))) {
+            f = of;
+            who = ow;
+        }
+    }
+    return __builtin_amdgcn_readlane(c, who);
 }
 
-__device__ void put_recipe(Recipe *rec, int o, int kind, int a, int pa, int b, int pb, int glen) {
+// Lane 0 stores the recipe as two 16-byte vectors (a struct assembled in
+// place would go through private scratch memory).
+__device__ __forceinline__ void put_recipe(Recipe *rec, int o, int kind, int a, int pa, int sa, int b, int pb,
+                                           int sb, int L) {
     if (threadIdx.x == 0) {
-        Recipe q;
-        q.kind = kind;
-        q.a = a;
-        q.pa = pa;
-        q.b = b;
-        q.pb = pb;
-        q.glen = glen;
-        rec[o] = q;
+        int4 *q = reinterpret_cast<int4 *>(rec + o);
+        q[0] = make_int4(kind, a, pa, sa);
+        q[1] = make_int4(b, pb, sb, L);
     }
 }
 
@@ -319,17 +339,17 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
                                                   Parents T, Recipe *recipes, int16_t *grow) {
     __shared__ uint32_t mt[624];
     __shared__ int pool[MAX_POOL];
-    __shared__ int res[MAX_TOURN];
-    __shared__ int gst[2 * MAX_DEPTH + 4];
     extern __shared__ double sh[];
-    double *fit = sh;                                   // pop
-    unsigned char *taken = (unsigned char *)(sh + P.pop);   // pop
+    double *fit = sh;                                        // pop
+    uint32_t *tws = (uint32_t *)(sh + P.pop);                // pop: the parents' tree words
+    unsigned char *taken = (unsigned char *)(tws + P.pop);   // pop
     const int r = (int)blockIdx.x;
     const int lane = (int)threadIdx.x;
     const size_t base = (size_t)r * P.pop;
     Recipe *rec = recipes + (size_t)r * P.n_off;
     for (int j = lane; j < P.pop; j += 64) {
         fit[j] = fitness[base + j];
+        tws[j] = T.tw[base + j];
         taken[j] = 0;
     }
     Rng g;
@@ -357,42 +377,46 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
         __syncthreads();
         if (lane == 0) taken[bi] = 1;
         __syncthreads();
-        put_recipe(rec, o++, R_COPY, bi, 0, 0, 0, 0);
+        put_recipe(rec, o++, R_COPY, bi, 0, 0, 0, 0, 0, (int)(tws[bi] & 0xffffu));
     }
     while (o < P.n_off) {
         const double u = g.random();
         if (u < P.p_cx) {
-            const int a = tournament(P, g, fit, res, pool);
-            const int b = tournament(P, g, fit, res, pool);
+            const int a = tournament(P, g, fit, pool);
+            const int b = tournament(P, g, fit, pool);
             const size_t ga = base + a, gb = base + b;
-            const int pa = select_point(T, ga, g);
-            const int pb = select_point(T, gb, g);
-            const int da = T.depth[ga * MAXL + pa], db = T.depth[gb * MAXL + pb];
-            const int ha = T.height[ga * MAXL + pa], hb = T.height[gb * MAXL + pb];
-            const int d1 = max(outside_depth(T, ga, pa), da + hb);
-            const int d2 = max(outside_depth(T, gb, pb), db + ha);
-            if (d1 > P.depth_limit) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0);
-            else put_recipe(rec, o, R_XOVER, a, pa, b, pb, 0);
+            const uint32_t ta = tws[a], tb = tws[b];
+            const uint32_t wa = T.pts[ga * MAXL + select_point(ta, g)];
+            const uint32_t wb = T.pts[gb * MAXL + select_point(tb, g)];
+            const int La = (int)(ta & 0xffffu), Lb = (int)(tb & 0xffffu);
+            const int pa = pw_pos(wa), pb = pw_pos(wb), sa = pw_size(wa), sb = pw_size(wb);
+            const int d1 = max(pw_outside(wa), pw_depth(wa) + pw_height(wb));
+            const int d2 = max(pw_outside(wb), pw_depth(wb) + pw_height(wa));
+            if (d1 > P.depth_limit) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0, 0, La);
+            else put_recipe(rec, o, R_XOVER, a, pa, sa, b, pb, sb, La - sa + sb);
             ++o;
             if (o < P.n_off) {
-                if (d2 > P.depth_limit) put_recipe(rec, o, R_COPY, b, 0, 0, 0, 0);
-                else put_recipe(rec, o, R_XOVER, b, pb, a, pa, 0);
+                if (d2 > P.depth_limit) put_recipe(rec, o, R_COPY, b, 0, 0, 0, 0, 0, Lb);
+                else put_recipe(rec, o, R_XOVER, b, pb, sb, a, pa, sa, Lb - sb + sa);
                 ++o;
             }
         } else if (u < P.p_cxmut) {
             // engine/gp.hpp:146-153
-            const int a = tournament(P, g, fit, res, pool);
+            const int a = tournament(P, g, fit, pool);
             const size_t ga = base + a;
-            const int pa = select_point(T, ga, g);
+            const uint32_t ta = tws[a];
+            const uint32_t wa = T.pts[ga * MAXL + select_point(ta, g)];
             const int nd = P.min_depth + (int)g.below((uint32_t)(P.max_depth - P.min_depth + 1));
             int h;
-            const int glen = generate(P, g, gst, nd, false, grow + ((size_t)r * P.n_off + o) * MAXL, h);
-            const int d = max(outside_depth(T, ga, pa), (int)T.depth[ga * MAXL + pa] + h);
-            if (d > P.max_depth) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0);
-            else put_recipe(rec, o, R_MUT, a, pa, 0, 0, glen);
+            const int glen = generate(P, g, nd, false, grow + ((size_t)r * P.n_off + o) * MAXL, h);
+            const int La = (int)(ta & 0xffffu), pa = pw_pos(wa), sa = pw_size(wa);
+            const int d = max(pw_outside(wa), pw_depth(wa) + h);
+            if (d > P.max_depth) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0, 0, La);
+            else put_recipe(rec, o, R_MUT, a, pa, sa, 0, 0, glen, La - sa + glen);
             ++o;
         } else {
-            put_recipe(rec, o, R_COPY, tournament(P, g, fit, res, pool), 0, 0, 0, 0);
+            const int a = tournament(P, g, fit, pool);
+            put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0, 0, (int)(tws[a] & 0xffffu));
             ++o;
         }
     }
@@ -402,31 +426,27 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
 // Offspring from recipes, one wavefront each: the prefix of parent a up to
 // the point, the donor subtree, the rest of a.
 __global__ __launch_bounds__(64) void assemble_kernel(GpParams P, const Recipe *recipes, const int16_t *ptok,
-                                                      Parents T, const int16_t *grow, int16_t *otok,
-                                                      int32_t *olen) {
+                                                      const int16_t *grow, int16_t *otok, int32_t *olen) {
     const int r = (int)blockIdx.y, o = (int)blockIdx.x;
     const size_t slot = (size_t)r * P.n_off + o;
     const Recipe q = recipes[slot];
-    const size_t ga = (size_t)r * P.pop + q.a;
-    const int16_t *A = ptok + ga * MAXL;
+    const int16_t *A = ptok + ((size_t)r * P.pop + q.a) * MAXL;
     int16_t *out = otok + slot * MAXL;
-    const int La = T.len[ga];
-    int L = La, pa = La, sa = 0, pb = 0, sb = 0;
+    const int L = q.L;   // <= 511: the depth checks kept every child within depth 8
+    int pa = L, sa = 0, pb = 0, sb = 0;
     const int16_t *Bt = A;
     if (q.kind == R_XOVER) {
-        const size_t gb = (size_t)r * P.pop + q.b;
         pa = q.pa;
+        sa = q.sa;
         pb = q.pb;
-        sa = T.size[ga * MAXL + pa];
-        sb = T.size[gb * MAXL + pb];
-        Bt = ptok + gb * MAXL;
+        sb = q.sb;
+        Bt = ptok + ((size_t)r * P.pop + q.b) * MAXL;
     } else if (q.kind == R_MUT) {
         pa = q.pa;
-        sa = T.size[ga * MAXL + pa];
-        sb = q.glen;
+        sa = q.sa;
+        sb = q.sb;
         Bt = grow + slot * MAXL;
     }
-    L = La - sa + sb;   // <= 511: the depth checks kept every child within depth 8
     for (int k = (int)threadIdx.x; k < L; k += 64)
         out[k] = k < pa ? A[k] : (k < pa + sb ? Bt[pb + k - pa] : A[k - sb + sa]);
     if (threadIdx.x == 0) olen[slot] = L;
@@ -444,13 +464,13 @@ __global__ __launch_bounds__(64) void gather_kernel(GpParams P, const int32_t *s
     if (threadIdx.x == 0) plen[dst] = L;
 }
 
-// Per tree, one wavefront: the breeding tables (when T.size is set) and the
+// Per tree, one wavefront: the breeding tables (when T.tw is set) and the
 // compiled postfix arrays (engine/vocab.hpp:96-125 compile_rmod, and the
 // batch preparation of the evaluator: dead leaves become the constant 0,
 // live ones index the compact value store).
 __global__ __launch_bounds__(64) void analyze_kernel(GpParams P, TreeSet T, uint64_t mask_bits) {
     __shared__ int16_t stok[MAXL], ssize[MAXL];
-    __shared__ int8_t sdep[MAXL], shei[MAXL];
+    __shared__ int8_t sdep[MAXL], shei[MAXL], spmax[MAXL], ssmax[MAXL];
     __shared__ int stk_pos[MAX_DEPTH + 4], stk_rem[MAX_DEPTH + 4], stk_h[MAX_DEPTH + 4];
     __shared__ uint64_t s_used;
     __shared__ int s_nint;
@@ -488,31 +508,26 @@ __global__ __launch_bounds__(64) void analyze_kernel(GpParams P, TreeSet T, uint
                 --sp;
             }
         }
+        if (T.tw) {
+            int m = -1;
+            for (int k = 0; k < L; ++k) {
+                m = max(m, (int)sdep[k]);
+                spmax[k] = (int8_t)m;
+            }
+            m = -1;
+            for (int k = L - 1; k >= 0; --k) {
+                m = max(m, (int)sdep[k]);
+                ssmax[k] = (int8_t)m;
+            }
+            T.tw[gi] = (uint32_t)L | ((uint32_t)nint << 16);
+        }
         s_used = used;
         s_nint = nint;
     }
     __syncthreads();
     const int nint = s_nint;
-    if (T.size) {
-        if (lane == 0) {
-            int m = -1;
-            for (int k = 0; k < L; ++k) {
-                m = max(m, (int)sdep[k]);
-                T.pmax[gi * MAXL + k] = (int8_t)m;
-            }
-            m = -1;
-            for (int k = L - 1; k >= 0; --k) {
-                m = max(m, (int)sdep[k]);
-                T.smax[gi * MAXL + k] = (int8_t)m;
-            }
-            T.nint[gi] = nint;
-        }
-        for (int k = lane; k < L; k += 64) {
-            T.size[gi * MAXL + k] = ssize[k];
-            T.depth[gi * MAXL + k] = sdep[k];
-            T.height[gi * MAXL + k] = shei[k];
-        }
-        // pre-order positions: internal nodes first, then leaves, each ascending
+    if (T.tw) {
+        // point words in pre-order, internal nodes first, then leaves
         int ci = 0, cl = 0;
         const uint64_t below_me = (1ull << lane) - 1;
         for (int b = 0; b < L; b += 64) {
@@ -520,8 +535,15 @@ __global__ __launch_bounds__(64) void analyze_kernel(GpParams P, TreeSet T, uint
             const bool in = k < L;
             const bool isf = in && (stok[k] & TOK_FUNC);
             const uint64_t mf = __ballot(isf), ml = __ballot(in && !isf);
-            if (isf) T.pos[gi * MAXL + ci + __popcll(mf & below_me)] = (int16_t)k;
-            if (in && !isf) T.pos[gi * MAXL + nint + cl + __popcll(ml & below_me)] = (int16_t)k;
+            if (in) {
+                const int e = k + ssize[k];
+                int out = k > 0 ? spmax[k - 1] : -1;   // depth outside the subtree at k
+                if (e < L) out = max(out, (int)ssmax[e]);
+                const uint32_t w = (uint32_t)ssize[k] | ((uint32_t)sdep[k] << 9) | ((uint32_t)shei[k] << 13) |
+                                   ((uint32_t)(out + 1) << 17) | ((uint32_t)k << 21);
+                const int slot = isf ? ci + __popcll(mf & below_me) : nint + cl + __popcll(ml & below_me);
+                T.pts[gi * MAXL + slot] = w;
+            }
             ci += __popcll(mf);
             cl += __popcll(ml);
         }
@@ -683,14 +705,8 @@ static sigp_status gp_fail(const std::string &m) { return sigp_internal_fail(SIG
 
 static Parents parents_of(const sigp_gp *g) {
     Parents T;
-    T.len = g->pset.len;
-    T.nint = g->pset.nint;
-    T.size = g->pset.size;
-    T.pos = g->pset.pos;
-    T.depth = g->pset.depth;
-    T.height = g->pset.height;
-    T.pmax = g->pset.pmax;
-    T.smax = g->pset.smax;
+    T.tw = g->pset.tw;
+    T.pts = g->pset.pts;
     return T;
 }
 
@@ -701,10 +717,10 @@ static void launch_rebuild(sigp_gp *g) {
 }
 
 static void launch_step(sigp_gp *g) {
-    const size_t lds = (size_t)g->prm.pop * (sizeof(double) + 1);
+    const size_t lds = (size_t)g->prm.pop * (sizeof(double) + 4 + 1);
     plan_kernel<<<g->runs, 64, lds, g->stream>>>(g->prm, g->mt, g->fit, parents_of(g), g->rec, g->grow);
-    assemble_kernel<<<dim3(g->prm.n_off, g->runs), 64, 0, g->stream>>>(g->prm, g->rec, g->ptok, parents_of(g),
-                                                                        g->grow, g->otok, g->olen);
+    assemble_kernel<<<dim3(g->prm.n_off, g->runs), 64, 0, g->stream>>>(g->prm, g->rec, g->ptok, g->grow,
+                                                                        g->otok, g->olen);
     gather_kernel<<<dim3(g->prm.pop, g->runs), 64, 0, g->stream>>>(g->prm, nullptr, g->otok, g->olen, g->ptok,
                                                                     g->plen);
     launch_rebuild(g);
@@ -790,8 +806,7 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     };
     const size_t o_mt = take((size_t)g->runs * MT_WORDS * 4), o_ptok = take(NL * 2), o_otok = take(OL * 2),
                  o_grow = take(OL * 2), o_plen = take(P * 4), o_olen = take(O * 4), o_rec = take(O * sizeof(Recipe)),
-                 o_size = take(NL * 2), o_pos = take(NL * 2), o_dep = take(NL), o_hei = take(NL), o_pmax = take(NL),
-                 o_smax = take(NL), o_nint = take(P * 4), o_rops = take(NL * 4), o_rargs = take(NL * 8),
+                 o_tw = take(P * 4), o_pts = take(NL * 4), o_rops = take(NL * 4), o_rargs = take(NL * 8),
                  o_ops = take(NL * 4), o_args = take(NL * 8), o_off = take(P * 4), o_len = take(P * 4),
                  o_rd = take(P * 8), o_fl = take(P * 8), o_cost = take(P * 4), o_ord = take(P * 4),
                  o_fit = take(P * 8), o_pairs = take(P * (size_t)g->T * 8),
@@ -811,14 +826,8 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     TreeSet &S = g->pset;
     S.tok = g->ptok;
     S.len = g->plen;
-    S.count = g->P;
-    S.size = (int16_t *)(b + o_size);
-    S.pos = (int16_t *)(b + o_pos);
-    S.depth = (int8_t *)(b + o_dep);
-    S.height = (int8_t *)(b + o_hei);
-    S.pmax = (int8_t *)(b + o_pmax);
-    S.smax = (int8_t *)(b + o_smax);
-    S.nint = (int32_t *)(b + o_nint);
+    S.tw = (uint32_t *)(b + o_tw);
+    S.pts = (uint32_t *)(b + o_pts);
     S.raw_ops = (int32_t *)(b + o_rops);
     S.raw_args = (int32_t *)(b + o_rargs);
     S.ops = (int32_t *)(b + o_ops);

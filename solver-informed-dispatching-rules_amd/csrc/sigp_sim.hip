@@ -1301,26 +1301,30 @@ static int fit_waves(const sigp_batch *bt, int w) {
 }
 
 // A size class's tuned mapping, else the batch's; AUTO: candidate lanes below
-// 16 scenarios per rule (scenario lanes would be mostly idle) or when
-// scenario lanes would make fewer than 512 workgroups (too few to fill 256
-// CUs, e.g. one rule scored on a held-out set), scenario lanes otherwise.
+// 64 scenarios per rule (less than one full block of scenario lanes; the GP
+// loop's shapes measured 64 ms with candidate lanes against about 300 ms with
+// scenario lanes at 50 scenarios) or when scenario lanes would make fewer
+// than 512 workgroups (too few to fill 256 CUs, e.g. one rule scored on a
+// held-out set), scenario lanes otherwise.
 static int resolved_mapping(const sigp_batch *bt, const LaunchUnit &u) {
     if (u.mapping != SIGP_MAP_AUTO) return u.mapping;
     if (bt->mapping != SIGP_MAP_AUTO) return bt->mapping;
     const long long scen_items = u.n_pairs * ((bt->T + 63) / 64);
-    return (bt->T < 16 || scen_items < 512) ? SIGP_MAP_CANDIDATES : SIGP_MAP_SCENARIOS;
+    return (bt->T < 64 || scen_items < 512) ? SIGP_MAP_CANDIDATES : SIGP_MAP_SCENARIOS;
 }
 
 // Wavefronts per workgroup: the tuned or configured count, else 4 for
-// scenario lanes and enough 64-candidate groups to cover the class's largest
-// instance (at most 4) for candidate lanes; then reduced until LDS fits.
+// scenario lanes; for candidate lanes one wavefront up to 128 customers (two
+// passes of 64 candidates measured faster than two half-idle wavefronts at
+// n = 100), else enough 64-candidate groups to cover the class's largest
+// instance (at most 4); then reduced until LDS fits.
 static int resolved_waves(const sigp_batch *bt, const LaunchUnit &u, int mapping) {
     int w = u.waves ? u.waves : bt->waves_per_item;
     if (w == 0) {
         w = 4;
         if (mapping == SIGP_MAP_CANDIDATES) {
             const int need = (u.max_n + 63) / 64;
-            w = need <= 1 ? 1 : (need <= 2 ? 2 : 4);
+            w = u.max_n <= 128 ? 1 : (need <= 2 ? 2 : 4);
         }
     }
     return fit_waves(bt, w);
