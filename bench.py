@@ -151,7 +151,8 @@ def bench_gp_loop(args):
     gp = DeviceGP(ctx, tc, ec, seeds=seeds, T=T, term_mask=mask, **launch)
     gp.init()
     wall0 = time.perf_counter()
-    hist, dev_ms = gp.run(rows, generations=gens, graph=not args.no_graph)
+    res = gp.run(rows, generations=gens, graph=not args.no_graph)
+    hist, dev_ms = res.fitness, res.ms
     wall = time.perf_counter() - wall0
     log("device loop: %d generations in %.1f ms device time, %.3f s wall" % (gens, dev_ms, wall))
     # the reference's read-out: best of the final population on one more

@@ -267,7 +267,8 @@ typedef struct sigp_gp sigp_gp;
 typedef struct {
     int32_t n_runs;                  /* independent runs evolved side by side (the reference's seeds) */
     int32_t population;              /* EvolutionConfig::population_size, <= 4096 */
-    int32_t offspring;               /* offspring per run and step: 0 = population */
+    int32_t offspring;               /* offspring per run and step: 0 = population; a multiple k >= 2 of it
+                                        screens them with sigp_gp_set_surrogate's network */
     int32_t n_terminals;             /* TreeConfig::terminals, <= 128 */
     const int32_t *terminal_slots;   /* slot of each terminal (0..60), -1 = "ERC" */
     int32_t n_functions;             /* TreeConfig::functions, <= 8 */
@@ -282,7 +283,9 @@ typedef struct {
 } sigp_gp_config;
 
 /* seeds: n_runs Mt19937 seeds; term_mask: int64[SIGP_N_TERMS] of the
- * evaluation (engine/gp_engine.cu:93). */
+ * evaluation (engine/gp_engine.cu:93).  A screened run (offspring = k x
+ * population) breeds k gp_steps per generation, step s on the stream
+ * Mt19937(seed + s * 2^32) (s = 0: the reference's stream). */
 sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config *cfg, const uint64_t *seeds,
                            const int64_t *term_mask, sigp_gp **out);
 sigp_status sigp_gp_destroy(sigp_gp *gp);
@@ -295,22 +298,34 @@ sigp_status sigp_gp_evaluate(sigp_gp *gp, const double *sampled, int64_t n_rows,
 /* One gp_step of every run (engine/gp.hpp:170-194) with the given host
  * fitness (n_runs*population), or NULL for the last sigp_gp_evaluate. */
 sigp_status sigp_gp_step(sigp_gp *gp, const double *fitness);
+/* Surrogate pre-screening (no counterpart in the reference) for a run
+ * created with offspring = k x population: each generation's k x population
+ * offspring are compiled on the device, their phenotypes taken on `sit`
+ * (n_sit x n_cand = SIGP_MLP_IN, as sigp_phenotype) and scored by `net`
+ * (not owned; it must outlive the run); the next parents are the elites
+ * and then the population - elitism best-scored offspring outside the
+ * elites' slots, best first, equal scores by slot. */
+sigp_status sigp_gp_set_surrogate(sigp_gp *gp, sigp_mlp *net, const sigp_situations *sit);
 /* The whole loop on the device: for g < generations, evaluate on rows
  * g*n_runs*T .. (n_rows = generations*n_runs*T) and step, except after the
  * last.  use_graph != 0 captures one generation as a HIP graph and replays
- * it.  fitness_history (nullable): generations x n_runs x population;
- * elapsed_ms (nullable): device time of the loop. */
+ * it.  Outputs (each nullable): fitness_history, generations x n_runs x
+ * population; best_tokens / best_lengths, each generation's and run's best
+ * rule (first maximum of the fitness), generations x n_runs x
+ * SIGP_GP_MAX_NODES tokens and lengths; generation_ms, the device time from
+ * the loop's start to the end of each generation. */
 sigp_status sigp_gp_run(sigp_gp *gp, const double *sampled, int64_t n_rows, int32_t generations,
-                        int32_t use_graph, double *fitness_history, double *elapsed_ms);
+                        int32_t use_graph, double *fitness_history, int16_t *best_tokens,
+                        int32_t *best_lengths, double *generation_ms);
 /* The population: tokens (n_runs*population*SIGP_GP_MAX_NODES) and lengths. */
 sigp_status sigp_gp_population(const sigp_gp *gp, int16_t *tokens, int32_t *lengths);
 /* The population compiled as pack_trees would (engine/factory.cuh:20-40),
  * tree i at i*SIGP_GP_MAX_NODES: ops (n_runs*population*SIGP_GP_MAX_NODES),
  * args (2x that), lengths. */
 sigp_status sigp_gp_packed(const sigp_gp *gp, int32_t *ops, int32_t *args, int32_t *lengths);
-/* Generator state of one run: 624 words and the position, as CPython's
- * random.getstate()[1]. */
-sigp_status sigp_gp_rng_state(const sigp_gp *gp, int32_t run, uint32_t *state);
+/* Generator state of one run's sub-step stream (0 = the reference's): 624
+ * words and the position, as CPython's random.getstate()[1]. */
+sigp_status sigp_gp_rng_state(const sigp_gp *gp, int32_t run, int32_t sub, uint32_t *state);
 
 #ifdef __cplusplus
 }

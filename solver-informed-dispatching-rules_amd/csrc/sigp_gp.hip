@@ -67,7 +67,8 @@ struct GpParams {
     int min_depth, max_depth, depth_limit;
     int tourn, elitism;
     int pop;                // parents per run
-    int n_off;              // offspring per run and step
+    int n_off;              // offspring per run and step: subs x pop
+    int subs;               // sub-steps per run and step, each a gp_step of pop offspring on its own stream
     int setsize;            // CPython sample(): pool method when pop <= setsize
     double p_cx, p_cxmut;   // crossover_rate, crossover_rate + mutation_rate
     double p_term;          // n_terminals / (n_terminals + n_functions)
@@ -248,12 +249,12 @@ __device__ __forceinline__ void store_rng(const Rng &g, uint32_t *state) {
 }
 
 // engine/gp.hpp:67-77: depths ramp min..max, alternating full/grow blocks.
-// One wavefront per run.
+// One wavefront per run, on the run's first stream.
 __global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, int16_t *tok, int32_t *len) {
     __shared__ uint32_t mt[624];
     const int r = (int)blockIdx.x;
     Rng g;
-    load_rng(g, mt, state + (size_t)r * MT_WORDS);
+    load_rng(g, mt, state + (size_t)r * P.subs * MT_WORDS);
     const int nd = P.max_depth - P.min_depth + 1;
     for (int i = 0; i < P.pop; ++i) {
         const int d = P.min_depth + i % nd;
@@ -263,7 +264,7 @@ __global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, i
         const int n = generate(P, g, d, full, tok + gi * MAXL, h);
         if (threadIdx.x == 0) len[gi] = n;
     }
-    store_rng(g, state + (size_t)r * MT_WORDS);
+    store_rng(g, state + (size_t)r * P.subs * MT_WORDS);
 }
 
 // ---------------------------------------------------------------------------
@@ -334,7 +335,9 @@ __device__ __forceinline__ void put_recipe(Recipe *rec, int o, int kind, int a, 
 // One generation step of every run (engine/gp.hpp:170-194): the elites in
 // the order of the reference's stable descending sort, then crossover (two
 // children, the second only while room remains), mutation or reproduction by
-// rate until n_off offspring exist.  One wavefront per run.
+// rate until pop offspring exist.  One wavefront per (run, sub-step); a
+// screened step breeds subs x pop offspring as subs such steps on their own
+// generator streams, sub-step 0 being the reference's.
 __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, const double *fitness,
                                                   Parents T, Recipe *recipes, int16_t *grow) {
     __shared__ uint32_t mt[624];
@@ -343,20 +346,21 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
     double *fit = sh;                                        // pop
     uint32_t *tws = (uint32_t *)(sh + P.pop);                // pop: the parents' tree words
     unsigned char *taken = (unsigned char *)(tws + P.pop);   // pop
-    const int r = (int)blockIdx.x;
+    const int r = (int)blockIdx.x / P.subs, sub = (int)blockIdx.x - r * P.subs;
     const int lane = (int)threadIdx.x;
     const size_t base = (size_t)r * P.pop;
-    Recipe *rec = recipes + (size_t)r * P.n_off;
+    const size_t first = (size_t)r * P.n_off + (size_t)sub * P.pop;   // this sub-step's offspring slots
+    Recipe *rec = recipes + first;
     for (int j = lane; j < P.pop; j += 64) {
         fit[j] = fitness[base + j];
         tws[j] = T.tw[base + j];
         taken[j] = 0;
     }
     Rng g;
-    load_rng(g, mt, state + (size_t)r * MT_WORDS);
+    load_rng(g, mt, state + (size_t)blockIdx.x * MT_WORDS);
 
     int o = 0;
-    const int E = min(P.elitism, min(P.pop, P.n_off));
+    const int E = min(P.elitism, P.pop);
     for (int e = 0; e < E; ++e) {
         double bf = 0.0;
         int bi = -1;
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
         __syncthreads();
         put_recipe(rec, o++, R_COPY, bi, 0, 0, 0, 0, 0, (int)(tws[bi] & 0xffffu));
     }
-    while (o < P.n_off) {
+    while (o < P.pop) {
         const double u = g.random();
         if (u < P.p_cx) {
             const int a = tournament(P, g, fit, pool);
@@ -395,7 +399,7 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
             if (d1 > P.depth_limit) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0, 0, La);
             else put_recipe(rec, o, R_XOVER, a, pa, sa, b, pb, sb, La - sa + sb);
             ++o;
-            if (o < P.n_off) {
+            if (o < P.pop) {
                 if (d2 > P.depth_limit) put_recipe(rec, o, R_COPY, b, 0, 0, 0, 0, 0, Lb);
                 else put_recipe(rec, o, R_XOVER, b, pb, sb, a, pa, sa, Lb - sb + sa);
                 ++o;
@@ -408,7 +412,7 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
             const uint32_t wa = T.pts[ga * MAXL + select_point(ta, g)];
             const int nd = P.min_depth + (int)g.below((uint32_t)(P.max_depth - P.min_depth + 1));
             int h;
-            const int glen = generate(P, g, nd, false, grow + ((size_t)r * P.n_off + o) * MAXL, h);
+            const int glen = generate(P, g, nd, false, grow + (first + o) * MAXL, h);
             const int La = (int)(ta & 0xffffu), pa = pw_pos(wa), sa = pw_size(wa);
             const int d = max(pw_outside(wa), pw_depth(wa) + h);
             if (d > P.max_depth) put_recipe(rec, o, R_COPY, a, 0, 0, 0, 0, 0, La);
@@ -420,7 +424,7 @@ __global__ __launch_bounds__(64) void plan_kernel(GpParams P, uint32_t *state, c
             ++o;
         }
     }
-    store_rng(g, state + (size_t)r * MT_WORDS);
+    store_rng(g, state + (size_t)blockIdx.x * MT_WORDS);
 }
 
 // Offspring from recipes, one wavefront each: the prefix of parent a up to
@@ -462,6 +466,72 @@ __global__ __launch_bounds__(64) void gather_kernel(GpParams P, const int32_t *s
     const int L = olen[src];
     for (int k = (int)threadIdx.x; k < L; k += 64) ptok[dst * MAXL + k] = otok[src * MAXL + k];
     if (threadIdx.x == 0) plen[dst] = L;
+}
+
+// The next parents of a screened step: run r keeps its E elites (the first
+// slots of sub-step 0), then the pop - E offspring with the highest
+// predicted reward among the non-elite slots of all sub-steps, best first,
+// equal predictions by slot; a NaN prediction ranks as -infinity.  One
+// thread per slot counts the candidates ranked before it.
+__global__ __launch_bounds__(256) void select_kernel(GpParams P, const float *pred, int32_t *sel) {
+    __shared__ float tile[1024];
+    const int r = (int)blockIdx.y;
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int E = min(P.elitism, P.pop), K = P.pop - E;
+    const float *pr = pred + (size_t)r * P.n_off;
+    // candidates' keys; NaN marks a non-candidate (it compares false)
+    auto key = [&](int q) {
+        if (q % P.pop < E) return __builtin_nanf("");
+        const float v = pr[q];
+        return v != v ? -INFINITY : v;
+    };
+    const bool cand = j < P.n_off && j % P.pop >= E;
+    const float mine = cand ? key(j) : 0.0f;
+    int rank = 0;
+    for (int b = 0; b < P.n_off; b += 1024) {
+        for (int q = (int)threadIdx.x; q < 1024; q += (int)blockDim.x)
+            tile[q] = b + q < P.n_off ? key(b + q) : __builtin_nanf("");
+        __syncthreads();
+        const int m = min(1024, P.n_off - b);
+        for (int q = 0; q < m; ++q) {
+            const float v = tile[q];
+            rank += (v > mine || (v == mine && b + q < j)) ? 1 : 0;
+        }
+        __syncthreads();
+    }
+    if (cand && rank < K) sel[(size_t)r * P.pop + E + rank] = j;
+    if (blockIdx.x == 0)
+        for (int e = (int)threadIdx.x; e < E; e += (int)blockDim.x) sel[(size_t)r * P.pop + e] = e;
+}
+
+// Each run's best rule of this generation's evaluation (the first maximum
+// of the fitness, as engine/gp_engine.cu:107-109 picks it): its tokens and
+// length into the history at the generation counter.
+__global__ __launch_bounds__(64) void best_kernel(GpParams P, const double *fit, const int16_t *ptok,
+                                                  const int32_t *plen, const int *gen, int16_t *best_tok,
+                                                  int32_t *best_len) {
+    const int r = (int)blockIdx.x, lane = (int)threadIdx.x;
+    const size_t base = (size_t)r * P.pop;
+    double bf = 0.0;
+    int bi = -1;
+    for (int j = lane; j < P.pop; j += 64)
+        if (bi < 0 || fit[base + j] > bf) {
+            bf = fit[base + j];
+            bi = j;
+        }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double of = __shfl_xor(bf, off);
+        const int oi = __shfl_xor(bi, off);
+        if (oi >= 0 && (bi < 0 || of > bf || (of == bf && oi < bi))) {
+            bf = of;
+            bi = oi;
+        }
+    }
+    const size_t slot = (size_t)(*gen) * gridDim.x + r;
+    const int L = plen[base + bi];
+    for (int k = lane; k < L; k += 64) best_tok[slot * MAXL + k] = ptok[(base + bi) * MAXL + k];
+    if (lane == 0) best_len[slot] = L;
 }
 
 // Per tree, one wavefront: the breeding tables (when T.tw is set) and the
@@ -688,15 +758,22 @@ struct sigp_gp {
     uint64_t mask = 0;
     int P = 0;                        // runs x pop
     char *dbuf = nullptr;             // every device array below
-    uint32_t *mt = nullptr;           // runs x 625
+    uint32_t *mt = nullptr;           // runs x subs x 625
     int16_t *ptok = nullptr, *otok = nullptr, *grow = nullptr;
     int32_t *plen = nullptr, *olen = nullptr;
     Recipe *rec = nullptr;
     TreeSet pset{};                   // the parents: tables + compiled arrays
+    TreeSet oset{};                   // screened steps: the offspring's compiled arrays
     int32_t *order = nullptr;
     double *fit = nullptr, *pairs = nullptr, *scen = nullptr;
     int *gen = nullptr;
-    sigp_batch *bt = nullptr;
+    int32_t *ranks = nullptr, *sel = nullptr;   // screened steps: phenotypes, next parents
+    float *pred = nullptr;                      // screened steps: predicted rewards
+    sigp_batch *bt = nullptr;         // the parents' evaluation
+    sigp_batch *obt = nullptr;        // the offspring's phenotype
+    sigp_mlp *net = nullptr;          // the screen (not owned)
+    char *sbuf = nullptr;             // the decision situations
+    sigp_device_situations sit{};
     hipStream_t stream = nullptr;
     bool ready = false;               // a population exists
 };
@@ -716,14 +793,48 @@ static void launch_rebuild(sigp_gp *g) {
     order_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->pset.cost, g->P, g->order);
 }
 
-static void launch_step(sigp_gp *g) {
-    const size_t lds = (size_t)g->prm.pop * (sizeof(double) + 4 + 1);
-    plan_kernel<<<g->runs, 64, lds, g->stream>>>(g->prm, g->mt, g->fit, parents_of(g), g->rec, g->grow);
-    assemble_kernel<<<dim3(g->prm.n_off, g->runs), 64, 0, g->stream>>>(g->prm, g->rec, g->ptok, g->grow,
-                                                                        g->otok, g->olen);
-    gather_kernel<<<dim3(g->prm.pop, g->runs), 64, 0, g->stream>>>(g->prm, nullptr, g->otok, g->olen, g->ptok,
-                                                                    g->plen);
+// One step of every run: breeding, then (screened) the offspring's
+// compilation, phenotypes, predicted rewards and the choice of the next
+// parents, then the new parents' analysis.
+static sigp_status launch_step(sigp_gp *g) {
+    const GpParams &p = g->prm;
+    const size_t lds = (size_t)p.pop * (sizeof(double) + 4 + 1);
+    plan_kernel<<<g->runs * p.subs, 64, lds, g->stream>>>(p, g->mt, g->fit, parents_of(g), g->rec, g->grow);
+    assemble_kernel<<<dim3(p.n_off, g->runs), 64, 0, g->stream>>>(p, g->rec, g->ptok, g->grow, g->otok, g->olen);
+    const int32_t *sel = nullptr;
+    if (p.subs > 1) {
+        analyze_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(p, g->oset, g->mask);
+        sigp_status st = sigp_internal_phenotype_device(g->obt, g->sit, g->ranks, g->stream);
+        if (st != SIGP_OK) return st;
+        st = sigp_internal_mlp_predict_ranks(g->net, g->ranks, (int64_t)g->runs * p.n_off, g->sit.n_cand, g->pred,
+                                             g->stream);
+        if (st != SIGP_OK) return st;
+        select_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->pred, g->sel);
+        sel = g->sel;
+    }
+    gather_kernel<<<dim3(p.pop, g->runs), 64, 0, g->stream>>>(p, sel, g->otok, g->olen, g->ptok, g->plen);
     launch_rebuild(g);
+    GP_TRY(hipGetLastError());
+    return SIGP_OK;
+}
+
+static sigp_status make_rules_batch(sigp_gp *g, const TreeSet &S, int P, const int32_t *order, int KL, int SL,
+                                    bool rmod, bool rare, int T, int group_size, const sigp_launch_config &cfg,
+                                    sigp_batch **out) {
+    sigp_device_rules R;
+    R.ops = S.ops;
+    R.args = S.args;
+    R.off = S.off;
+    R.len = S.plen;
+    R.rd = S.rd;
+    R.flags = S.flags;
+    R.order = order;
+    R.P = P;
+    R.KL = KL;
+    R.SL = SL;
+    R.has_rmod = rmod;
+    R.has_rare = rare;
+    return sigp_internal_batch_device(g->inst, R, T, group_size, cfg, out);
 }
 
 extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config *c, const uint64_t *seeds,
@@ -735,8 +846,8 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     if (c->population < 1 || c->population > MAX_POP)
         return gp_fail("population must be in 1.." + std::to_string(MAX_POP));
     const int n_off = c->offspring == 0 ? c->population : c->offspring;
-    if (n_off != c->population)
-        return gp_fail("offspring must equal population (0) for the plain GP step");
+    if (n_off < c->population || n_off % c->population != 0 || n_off / c->population > 64)
+        return gp_fail("offspring must be 0 or a multiple (1..64) of the population");
     if (c->n_terminals < 1 || c->n_terminals > MAX_TERMS || !c->terminal_slots)
         return gp_fail("need 1.." + std::to_string(MAX_TERMS) + " terminals");
     if (c->n_functions < 1 || c->n_functions > MAX_FUNCS || !c->function_ops)
@@ -747,7 +858,7 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     if (c->tournament_size < 1 || c->tournament_size > MAX_TOURN) return gp_fail("tournament_size must be in 1..64");
     if (c->elitism < 0) return gp_fail("elitism < 0");
     if (c->T < 1) return gp_fail("T must be >= 1");
-    if ((long long)c->n_runs * c->population > (1 << 20)) return gp_fail("n_runs x population exceeds 2^20");
+    if ((long long)c->n_runs * n_off > (1 << 20)) return gp_fail("n_runs x offspring exceeds 2^20");
 
     GpParams p{};
     p.nt = c->n_terminals;
@@ -773,6 +884,7 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     p.elitism = c->elitism;
     p.pop = c->population;
     p.n_off = n_off;
+    p.subs = n_off / c->population;
     // CPython sample(): setsize = 21 (+ 4 ** ceil(log(3k, 4)) for k > 5)
     const int k = std::min(c->tournament_size, c->population);
     int setsize = 21;
@@ -797,20 +909,25 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     g->n = sigp_internal_instance_n(inst);
     g->mask = mk;
     g->P = c->n_runs * c->population;
+    const bool screened = p.subs > 1;
     const size_t P = (size_t)g->P, O = (size_t)c->n_runs * n_off, NL = P * MAXL, OL = O * MAXL;
+    const size_t OC = screened ? O : 0, OCL = OC * MAXL;   // the offspring's compiled arrays
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
         off += (bytes + 8 + 255) & ~(size_t)255;
         return o;
     };
-    const size_t o_mt = take((size_t)g->runs * MT_WORDS * 4), o_ptok = take(NL * 2), o_otok = take(OL * 2),
-                 o_grow = take(OL * 2), o_plen = take(P * 4), o_olen = take(O * 4), o_rec = take(O * sizeof(Recipe)),
-                 o_tw = take(P * 4), o_pts = take(NL * 4), o_rops = take(NL * 4), o_rargs = take(NL * 8),
-                 o_ops = take(NL * 4), o_args = take(NL * 8), o_off = take(P * 4), o_len = take(P * 4),
-                 o_rd = take(P * 8), o_fl = take(P * 8), o_cost = take(P * 4), o_ord = take(P * 4),
-                 o_fit = take(P * 8), o_pairs = take(P * (size_t)g->T * 8),
-                 o_scen = take((size_t)g->runs * g->T * g->n * 8), o_gen = take(4);
+    const size_t o_mt = take((size_t)g->runs * p.subs * MT_WORDS * 4), o_ptok = take(NL * 2),
+                 o_otok = take(OL * 2), o_grow = take(OL * 2), o_plen = take(P * 4), o_olen = take(O * 4),
+                 o_rec = take(O * sizeof(Recipe)), o_tw = take(P * 4), o_pts = take(NL * 4),
+                 o_rops = take(NL * 4), o_rargs = take(NL * 8), o_ops = take(NL * 4), o_args = take(NL * 8),
+                 o_off = take(P * 4), o_len = take(P * 4), o_rd = take(P * 8), o_fl = take(P * 8),
+                 o_cost = take(P * 4), o_ord = take(P * 4), o_fit = take(P * 8),
+                 o_pairs = take(P * (size_t)g->T * 8), o_scen = take((size_t)g->runs * g->T * g->n * 8),
+                 o_gen = take(4), o_oops = take(OCL * 4), o_oargs = take(OCL * 8), o_ooff = take(OC * 4),
+                 o_olen2 = take(OC * 4), o_ord2 = take(OC * 8), o_ofl = take(OC * 8), o_ocost = take(OC * 4),
+                 o_ranks = take(OC * SIGP_MLP_IN * 4), o_pred = take(OC * 4), o_sel = take(screened ? P * 4 : 0);
     if (hipMalloc((void **)&g->dbuf, off) != hipSuccess) {
         delete g;
         return sigp_internal_fail(SIGP_OOM, "GP allocation failed");
@@ -837,34 +954,41 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     S.rd = (uint64_t *)(b + o_rd);
     S.flags = (uint64_t *)(b + o_fl);
     S.cost = (float *)(b + o_cost);
+    TreeSet &Q = g->oset;
+    Q.tok = g->otok;
+    Q.len = g->olen;
+    Q.ops = (int32_t *)(b + o_oops);
+    Q.args = (int32_t *)(b + o_oargs);
+    Q.off = (int32_t *)(b + o_ooff);
+    Q.plen = (int32_t *)(b + o_olen2);
+    Q.rd = (uint64_t *)(b + o_ord2);
+    Q.flags = (uint64_t *)(b + o_ofl);
+    Q.cost = (float *)(b + o_ocost);
     g->order = (int32_t *)(b + o_ord);
     g->fit = (double *)(b + o_fit);
     g->pairs = (double *)(b + o_pairs);
     g->scen = (double *)(b + o_scen);
     g->gen = (int *)(b + o_gen);
+    g->ranks = (int32_t *)(b + o_ranks);
+    g->pred = (float *)(b + o_pred);
+    g->sel = (int32_t *)(b + o_sel);
 
-    std::vector<uint32_t> st((size_t)g->runs * MT_WORDS);
-    for (int r = 0; r < g->runs; ++r) mt_seed(seeds[r], st.data() + (size_t)r * MT_WORDS);
+    // run r's sub-step s draws from Mt19937(seeds[r] + s * 2^32)
+    std::vector<uint32_t> st((size_t)g->runs * p.subs * MT_WORDS);
+    for (int r = 0; r < g->runs; ++r)
+        for (int q = 0; q < p.subs; ++q)
+            mt_seed(seeds[r] + ((uint64_t)q << 32), st.data() + ((size_t)r * p.subs + q) * MT_WORDS);
     sigp_status status = SIGP_OK;
     if (hipMemcpy(g->mt, st.data(), st.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         status = sigp_internal_fail(SIGP_DEVICE_ERROR, "GP set-up failed");
-    if (status == SIGP_OK) {
-        sigp_device_rules R;
-        R.ops = S.ops;
-        R.args = S.args;
-        R.off = S.off;
-        R.len = S.plen;
-        R.rd = S.rd;
-        R.flags = S.flags;
-        R.order = g->order;
-        R.P = g->P;
-        R.KL = std::max(1, __builtin_popcountll(rd_all));
-        R.SL = std::max(1, std::max(c->max_depth, c->depth_limit));
-        R.has_rmod = rmod;
-        R.has_rare = (rd_all & RARE_SLOTS) != 0;
-        status = sigp_internal_batch_device(inst, R, g->T, c->population, c->launch, &g->bt);
-    }
+    const int KL = std::max(1, __builtin_popcountll(rd_all));
+    const int SL = std::max(1, std::max(c->max_depth, c->depth_limit));
+    const bool rare = (rd_all & RARE_SLOTS) != 0;
+    if (status == SIGP_OK)
+        status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, c->launch, &g->bt);
+    if (status == SIGP_OK && screened)
+        status = make_rules_batch(g, Q, (int)O, nullptr, KL, SL, rmod, rare, 1, (int)O, c->launch, &g->obt);
     if (status != SIGP_OK) {
         sigp_gp_destroy(g);
         return status;
@@ -873,23 +997,75 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     return SIGP_OK;
 }
 
+extern "C" sigp_status sigp_gp_set_surrogate(sigp_gp *g, sigp_mlp *net, const sigp_situations *sit) {
+    if (!g || !net || !sit) return gp_fail("NULL argument");
+    if (g->prm.subs < 2) return gp_fail("a screen needs offspring > population");
+    const int S = sit->n_sit, C = sit->n_cand, n = g->n;
+    if (S < 1 || C < 2 || (long long)S * C != SIGP_MLP_IN)
+        return gp_fail("need n_sit x n_cand = " + std::to_string(SIGP_MLP_IN) + " and n_cand >= 2");
+    if (!sit->cur || !sit->ct || !sit->remaining || !sit->visited || !sit->cand)
+        return gp_fail("situation arrays must be non-NULL");
+    std::vector<uint32_t> words((size_t)S * 16, 0u);
+    for (int q = 0; q < S; ++q) {
+        if (sit->cur[q] < 0 || sit->cur[q] > n) return gp_fail("situation cur out of range");
+        if (sit->remaining[q] < 0) return gp_fail("situation remaining < 0");
+        for (int i = 0; i < n; ++i)
+            if (sit->visited[(size_t)q * n + i]) words[(size_t)q * 16 + (i >> 5)] |= 1u << (i & 31);
+        for (int j = 0; j < C; ++j)
+            if (sit->cand[(size_t)q * C + j] < 0 || sit->cand[(size_t)q * C + j] >= n)
+                return gp_fail("candidate out of range");
+    }
+    GP_TRY(hipStreamSynchronize(g->stream));
+    if (g->sbuf) {
+        (void)hipFree(g->sbuf);
+        g->sbuf = nullptr;
+    }
+    const size_t o_cur = 0, o_rem = 256 * ((S * 4 + 255) / 256), o_cand = o_rem + 256 * ((S * 4 + 255) / 256),
+                 o_ct = o_cand + 256 * (((size_t)S * C * 4 + 255) / 256), o_vis = o_ct + 256 * ((S * 8 + 255) / 256),
+                 total = o_vis + words.size() * 4;
+    GP_TRY(hipMalloc((void **)&g->sbuf, total));
+    GP_TRY(hipMemcpy(g->sbuf + o_cur, sit->cur, S * 4, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(g->sbuf + o_rem, sit->remaining, S * 4, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(g->sbuf + o_cand, sit->cand, (size_t)S * C * 4, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(g->sbuf + o_ct, sit->ct, S * 8, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(g->sbuf + o_vis, words.data(), words.size() * 4, hipMemcpyHostToDevice));
+    g->sit.n_sit = S;
+    g->sit.n_cand = C;
+    g->sit.cur = (const int32_t *)(g->sbuf + o_cur);
+    g->sit.rem = (const int32_t *)(g->sbuf + o_rem);
+    g->sit.cand = (const int32_t *)(g->sbuf + o_cand);
+    g->sit.ct = (const double *)(g->sbuf + o_ct);
+    g->sit.vis = (const uint32_t *)(g->sbuf + o_vis);
+    g->net = net;
+    return SIGP_OK;
+}
+
 extern "C" sigp_status sigp_gp_destroy(sigp_gp *g) {
     if (!g) return SIGP_OK;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->bt) sigp_batch_destroy(g->bt);
+    if (g->obt) sigp_batch_destroy(g->obt);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     if (g->dbuf) (void)hipFree(g->dbuf);
+    if (g->sbuf) (void)hipFree(g->sbuf);
     delete g;
     return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_gp_init(sigp_gp *g) {
     if (!g) return gp_fail("gp is NULL");
+    // ramped half-and-half from each run's first stream
     init_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->mt, g->ptok, g->plen);
     launch_rebuild(g);
     GP_TRY(hipGetLastError());
     GP_TRY(hipStreamSynchronize(g->stream));
     g->ready = true;
+    return SIGP_OK;
+}
+
+static sigp_status check_step(const sigp_gp *g) {
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    if (g->prm.subs > 1 && !g->net) return gp_fail("screened steps need sigp_gp_set_surrogate first");
     return SIGP_OK;
 }
 
@@ -908,89 +1084,99 @@ extern "C" sigp_status sigp_gp_evaluate(sigp_gp *g, const double *sampled, int64
 
 extern "C" sigp_status sigp_gp_step(sigp_gp *g, const double *fitness) {
     if (!g) return gp_fail("gp is NULL");
-    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    sigp_status st = check_step(g);
+    if (st != SIGP_OK) return st;
     if (fitness) GP_TRY(hipMemcpyAsync(g->fit, fitness, (size_t)g->P * 8, hipMemcpyHostToDevice, g->stream));
-    launch_step(g);
-    GP_TRY(hipGetLastError());
+    if ((st = launch_step(g)) != SIGP_OK) return st;
     GP_TRY(hipStreamSynchronize(g->stream));
     return SIGP_OK;
 }
 
 // One generation of the loop on the stream: this generation's rows, the
-// evaluation and its record, and (unless last) the step to the next.
-static sigp_status launch_generation(sigp_gp *g, const double *all, double *hist, bool step) {
+// evaluation, its records, and (unless last) the step to the next.
+static sigp_status launch_generation(sigp_gp *g, const double *all, double *hist, int16_t *best_tok,
+                                     int32_t *best_len, bool step) {
     const long long per_gen = (long long)g->runs * g->T * g->n;
     rows_kernel<<<(unsigned)std::min<long long>((per_gen + 255) / 256, 1024), 256, 0, g->stream>>>(all, g->gen,
                                                                                                    per_gen, g->scen);
     sigp_status st = sigp_batch_run(g->bt, g->scen, (int64_t)g->runs * g->T, g->pairs, g->fit, g->stream);
     if (st != SIGP_OK) return st;
     record_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->fit, g->gen, g->P, hist);
-    if (step) launch_step(g);
+    best_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->fit, g->ptok, g->plen, g->gen, best_tok, best_len);
+    if (step && (st = launch_step(g)) != SIGP_OK) return st;
     tick_kernel<<<1, 64, 0, g->stream>>>(g->gen);
     GP_TRY(hipGetLastError());
     return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_rows, int32_t generations,
-                                   int32_t use_graph, double *fitness_history, double *elapsed_ms) {
+                                   int32_t use_graph, double *fitness_history, int16_t *best_tokens,
+                                   int32_t *best_lengths, double *generation_ms) {
     if (!g || !sampled) return gp_fail("NULL argument");
-    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    sigp_status st0 = check_step(g);
+    if (st0 != SIGP_OK) return st0;
     if (generations < 1) return gp_fail("generations must be >= 1");
     const long long rows_per_gen = (long long)g->runs * g->T;
     if (n_rows != rows_per_gen * generations)
         return gp_fail("need generations x n_runs x T = " + std::to_string(rows_per_gen * generations) + " rows");
-    const size_t sbytes = (size_t)n_rows * g->n * 8, hbytes = (size_t)generations * g->P * 8;
-    double *all = nullptr, *hist = nullptr;
-    GP_TRY(hipMalloc((void **)&all, sbytes));
-    if (hipMalloc((void **)&hist, hbytes) != hipSuccess) {
-        (void)hipFree(all);
-        return sigp_internal_fail(SIGP_OOM, "history allocation failed");
-    }
+    const size_t sbytes = (size_t)n_rows * g->n * 8, hbytes = (size_t)generations * g->P * 8,
+                 nbest = (size_t)generations * g->runs;
+    char *dev = nullptr;
+    const size_t o_hist = (sbytes + 255) & ~(size_t)255, o_btok = o_hist + ((hbytes + 255) & ~(size_t)255),
+                 o_blen = o_btok + ((nbest * MAXL * 2 + 255) & ~(size_t)255), total = o_blen + nbest * 4;
+    GP_TRY(hipMalloc((void **)&dev, total));
+    double *all = (double *)dev, *hist = (double *)(dev + o_hist);
+    int16_t *btok = (int16_t *)(dev + o_btok);
+    int32_t *blen = (int32_t *)(dev + o_blen);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<hipEvent_t> ev(generations + 1, nullptr);
     sigp_status st = SIGP_OK;
     auto check = [&](hipError_t e, const char *what) {
         if (e != hipSuccess && st == SIGP_OK)
             st = sigp_internal_fail(SIGP_DEVICE_ERROR, std::string(what) + ": " + hipGetErrorString(e));
         return st == SIGP_OK;
     };
-    if (check(hipMemcpyAsync(all, sampled, sbytes, hipMemcpyHostToDevice, g->stream), "upload") &&
-        check(hipEventCreate(&e0), "event") && check(hipEventCreate(&e1), "event")) {
+    for (auto &e : ev)
+        if (!check(hipEventCreate(&e), "event")) break;
+    if (st == SIGP_OK && check(hipMemcpyAsync(all, sampled, sbytes, hipMemcpyHostToDevice, g->stream), "upload")) {
         set_kernel<<<1, 64, 0, g->stream>>>(g->gen, 0);
         if (use_graph && generations > 1) {
             // capture one stepping generation; the counter it reads advances on the device
             if (check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "begin capture")) {
-                const sigp_status s2 = launch_generation(g, all, hist, true);
+                const sigp_status s2 = launch_generation(g, all, hist, btok, blen, true);
                 const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
                 if (s2 != SIGP_OK && st == SIGP_OK) st = s2;
                 if (check(ec, "end capture")) check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
             }
         }
-        if (st == SIGP_OK) check(hipEventRecord(e0, g->stream), "event");
+        if (st == SIGP_OK) check(hipEventRecord(ev[0], g->stream), "event");
         for (int gi = 0; st == SIGP_OK && gi < generations; ++gi) {
             const bool step = gi + 1 < generations;
             if (exec && step) check(hipGraphLaunch(exec, g->stream), "graph launch");
             else {
-                const sigp_status s2 = launch_generation(g, all, hist, step);
+                const sigp_status s2 = launch_generation(g, all, hist, btok, blen, step);
                 if (s2 != SIGP_OK) st = s2;
             }
+            if (st == SIGP_OK) check(hipEventRecord(ev[gi + 1], g->stream), "event");
         }
-        if (st == SIGP_OK) check(hipEventRecord(e1, g->stream), "event");
-        if (st == SIGP_OK && check(hipStreamSynchronize(g->stream), "run") && elapsed_ms) {
-            float ms = 0.0f;
-            if (check(hipEventElapsedTime(&ms, e0, e1), "timing")) *elapsed_ms = ms;
+        if (st == SIGP_OK && check(hipStreamSynchronize(g->stream), "run") && generation_ms) {
+            for (int gi = 0; gi < generations; ++gi) {
+                float ms = 0.0f;
+                if (!check(hipEventElapsedTime(&ms, ev[0], ev[gi + 1]), "timing")) break;
+                generation_ms[gi] = ms;
+            }
         }
-        if (st == SIGP_OK && fitness_history)
-            check(hipMemcpy(fitness_history, hist, hbytes, hipMemcpyDeviceToHost), "history download");
+        if (st == SIGP_OK && fitness_history) check(hipMemcpy(fitness_history, hist, hbytes, hipMemcpyDeviceToHost), "download");
+        if (st == SIGP_OK && best_tokens) check(hipMemcpy(best_tokens, btok, nbest * MAXL * 2, hipMemcpyDeviceToHost), "download");
+        if (st == SIGP_OK && best_lengths) check(hipMemcpy(best_lengths, blen, nbest * 4, hipMemcpyDeviceToHost), "download");
     }
     (void)hipStreamSynchronize(g->stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(all);
-    (void)hipFree(hist);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipFree(dev);
     return st;
 }
 
@@ -1013,10 +1199,11 @@ extern "C" sigp_status sigp_gp_packed(const sigp_gp *g, int32_t *ops, int32_t *a
     return SIGP_OK;
 }
 
-extern "C" sigp_status sigp_gp_rng_state(const sigp_gp *g, int32_t run, uint32_t *state) {
+extern "C" sigp_status sigp_gp_rng_state(const sigp_gp *g, int32_t run, int32_t sub, uint32_t *state) {
     if (!g || !state) return gp_fail("NULL argument");
-    if (run < 0 || run >= g->runs) return gp_fail("run out of range");
+    if (run < 0 || run >= g->runs || sub < 0 || sub >= g->prm.subs) return gp_fail("run or sub-step out of range");
     GP_TRY(hipStreamSynchronize(g->stream));
-    GP_TRY(hipMemcpy(state, g->mt + (size_t)run * MT_WORDS, MT_WORDS * 4, hipMemcpyDeviceToHost));
+    GP_TRY(hipMemcpy(state, g->mt + ((size_t)run * g->prm.subs + sub) * MT_WORDS, MT_WORDS * 4,
+                     hipMemcpyDeviceToHost));
     return SIGP_OK;
 }

@@ -43,3 +43,9 @@ sigp_status sigp_internal_phenotype_device(const sigp_batch *bt, const sigp_devi
                                            int32_t *out_dev, hipStream_t stream);
 
 int sigp_internal_instance_n(const sigp_instance *inst);
+
+// Surrogate scores of `rows` rules from device-resident phenotype ranks
+// (int32 [rows][SIGP_MLP_IN]) into y (device floats), on `stream`; the same
+// values sigp_mlp_predict gives for the host features rank / (n_cand - 1).
+sigp_status sigp_internal_mlp_predict_ranks(sigp_mlp *net, const int32_t *ranks, int64_t rows, int n_cand,
+                                            float *y, hipStream_t stream);

@@ -15,8 +15,8 @@
 #include <vector>
 
 #include "../../include/sigp.h"
+#include "sigp_internal.h"
 
-sigp_status sigp_internal_fail(sigp_status st, const std::string &msg);   // sigp_sim.hip
 #define MLP_TRY(x)                                                                                \
     do {                                                                                          \
         hipError_t e_ = (x);                                                                      \
@@ -300,25 +300,25 @@ extern "C" sigp_status sigp_mlp_set_params(sigp_mlp *n, const float *in) {
 
 // Forward pass (and, with targets, backward pass into n->g) of the batch
 // already in n->xb: nb live rows padded to B (a multiple of 16).
-static sigp_status run_batch(sigp_mlp *n, int nb, int B, bool backward) {
+static sigp_status run_batch(sigp_mlp *n, int nb, int B, bool backward, hipStream_t stream = nullptr) {
     const int tiles_h = (B / 16) * (H / 16);
     const int grid_h = (tiles_h + 3) / 4;
-    gemm_kernel<false, false, EPI_BIAS_RELU><<<grid_h, 256>>>(n->xb, IN, n->w1, H, B, H, IN, n->p + O_B1,
+    gemm_kernel<false, false, EPI_BIAS_RELU><<<grid_h, 256, 0, stream>>>(n->xb, IN, n->w1, H, B, H, IN, n->p + O_B1,
                                                               nullptr, n->h1, n->a1);
-    gemm_kernel<false, false, EPI_BIAS_RELU><<<grid_h, 256>>>(n->a1, H, n->w2, H, B, H, H, n->p + O_B2,
+    gemm_kernel<false, false, EPI_BIAS_RELU><<<grid_h, 256, 0, stream>>>(n->a1, H, n->w2, H, B, H, H, n->p + O_B2,
                                                               nullptr, n->h2, n->a2);
-    head_kernel<<<(B + 3) / 4, 256>>>(n->a2, n->h2, n->p + O_W3, n->p + O_B3, n->tb, nb, B, n->y, n->dy,
+    head_kernel<<<(B + 3) / 4, 256, 0, stream>>>(n->a2, n->h2, n->p + O_W3, n->p + O_B3, n->tb, nb, B, n->y, n->dy,
                                       n->se, n->d2, backward);
     if (backward) {
         // delta1 = (delta2 . W2^T) * (H1 > 0)
-        gemm_kernel<false, true, EPI_RELU_GRAD><<<grid_h, 256>>>(n->d2, H, n->w2, H, B, H, H, nullptr,
+        gemm_kernel<false, true, EPI_RELU_GRAD><<<grid_h, 256, 0, stream>>>(n->d2, H, n->w2, H, B, H, H, nullptr,
                                                                  n->h1, nullptr, n->d1);
         // dW2 = A1^T . delta2, dW1 = X^T . delta1
-        gemm_kernel<true, false, EPI_STORE><<<((H / 16) * (H / 16) + 3) / 4, 256>>>(
+        gemm_kernel<true, false, EPI_STORE><<<((H / 16) * (H / 16) + 3) / 4, 256, 0, stream>>>(
             n->a1, H, n->d2, H, H, H, B, nullptr, nullptr, n->g + O_W2, nullptr);
-        gemm_kernel<true, false, EPI_STORE><<<((IN / 16) * (H / 16) + 3) / 4, 256>>>(
+        gemm_kernel<true, false, EPI_STORE><<<((IN / 16) * (H / 16) + 3) / 4, 256, 0, stream>>>(
             n->xb, IN, n->d1, H, IN, H, B, nullptr, nullptr, n->g + O_W1, nullptr);
-        small_grads_kernel<<<1, 256>>>(n->a2, n->dy, n->d2, n->d1, B, n->g + O_W3, n->g + O_B3,
+        small_grads_kernel<<<1, 256, 0, stream>>>(n->a2, n->dy, n->d2, n->d1, B, n->g + O_W3, n->g + O_B3,
                                        n->g + O_B2, n->g + O_B1);
     }
     MLP_TRY(hipGetLastError());
@@ -326,6 +326,40 @@ static sigp_status run_batch(sigp_mlp *n, int nb, int B, bool backward) {
 }
 
 static int padded(int nb) { return (nb + 31) / 32 * 32; }
+
+// Phenotype ranks as network inputs: rank / (n_cand - 1) in FP32, rounded
+// to BF16 (what the host computes and uploads), rows [0, nb) of the batch and
+// zero rows up to B.
+__global__ void ranks_to_bf16_kernel(const int32_t *__restrict__ ranks, int nb, int B, int n_cand,
+                                     uint16_t *__restrict__ xb) {
+    const float den = (float)(n_cand > 1 ? n_cand - 1 : 1);
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < (size_t)B * IN;
+         k += (size_t)gridDim.x * blockDim.x)
+        xb[k] = k < (size_t)nb * IN ? to_bf16((float)ranks[k] / den) : (uint16_t)0;
+}
+
+__global__ void copy_out_kernel(const float *__restrict__ y, int nb, float *__restrict__ out) {
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (k < nb) out[k] = y[k];
+}
+
+// Scores of `rows` rules from their device-resident phenotype ranks
+// (int32 [rows][512]) into y (device), all on `stream`: the same features
+// and forward pass as sigp_mlp_predict, so the same values.
+sigp_status sigp_internal_mlp_predict_ranks(sigp_mlp *n, const int32_t *ranks, int64_t rows, int n_cand,
+                                            float *y, hipStream_t stream) {
+    for (int64_t r0 = 0; r0 < rows; r0 += MAXB) {
+        const int nb = (int)std::min<int64_t>(MAXB, rows - r0);
+        const int B = (nb + 31) / 32 * 32;
+        ranks_to_bf16_kernel<<<(B * IN + 255) / 256, 256, 0, stream>>>(ranks + (size_t)r0 * IN, nb, B, n_cand,
+                                                                       n->xb);
+        sigp_status st = run_batch(n, nb, B, false, stream);
+        if (st != SIGP_OK) return st;
+        copy_out_kernel<<<(nb + 255) / 256, 256, 0, stream>>>(n->y, nb, y + r0);
+    }
+    MLP_TRY(hipGetLastError());
+    return SIGP_OK;
+}
 
 // Uploads host features (converted to BF16) and targets for rows [0, nb).
 static sigp_status load_batch(sigp_mlp *n, const float *x, const float *t, int nb) {

@@ -23,7 +23,8 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_evaluate_multi", "sigp_phenotype", "sigp_mlp_create", "sigp_mlp_destroy",
             "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
             "sigp_mlp_predict", "sigp_gp_create", "sigp_gp_destroy", "sigp_gp_init", "sigp_gp_evaluate",
-            "sigp_gp_step", "sigp_gp_run", "sigp_gp_population", "sigp_gp_packed", "sigp_gp_rng_state"]
+            "sigp_gp_step", "sigp_gp_run", "sigp_gp_population", "sigp_gp_packed", "sigp_gp_rng_state",
+            "sigp_gp_set_surrogate"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
            4: "SIGP_NO_DEVICE"}
@@ -121,10 +122,11 @@ def lib() -> C.CDLL:
         "sigp_gp_init": ([vp], C.c_int),
         "sigp_gp_evaluate": ([vp, vp, i64, vp], C.c_int),
         "sigp_gp_step": ([vp, vp], C.c_int),
-        "sigp_gp_run": ([vp, vp, i64, i32, i32, vp, vp], C.c_int),
+        "sigp_gp_run": ([vp, vp, i64, i32, i32, vp, vp, vp, vp], C.c_int),
+        "sigp_gp_set_surrogate": ([vp, vp, vp], C.c_int),
         "sigp_gp_population": ([vp, vp, vp], C.c_int),
         "sigp_gp_packed": ([vp, vp, vp, vp], C.c_int),
-        "sigp_gp_rng_state": ([vp, i32, vp], C.c_int),
+        "sigp_gp_rng_state": ([vp, i32, i32, vp], C.c_int),
     }
     for name, (argt, rest) in sig.items():
         fn = getattr(L, name)

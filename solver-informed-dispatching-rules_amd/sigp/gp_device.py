@@ -11,22 +11,31 @@ generation captured as a HIP graph.
     gp = DeviceGP(ctx, TreeConfig(vocab), EvolutionConfig(), seeds=[0], T=50,
                   term_mask=term_mask_from(vocab))
     gp.init()
-    hist, ms = gp.run(all_sampled, generations=50)   # (G, runs, pop) fitness
-    best_rules = gp.population()
+    res = gp.run(all_sampled, generations=50)   # fitness history, best rules, times
+    final = gp.population()
+
+With `offspring` = k x population and a trained surrogate network
+(`set_surrogate`), each generation breeds k gp_steps (step s on the stream
+Mt19937(seed + s * 2**32)), and the network's phenotype scores choose the
+next parents among them: the elites, then the best-scored offspring.
 """
 from __future__ import annotations
 
 import ctypes as C
+from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
 from ._lib import GP_MAX_NODES, GP_TOK_CONST, GP_TOK_FUNC, MAP_AUTO, GpConfig, LaunchConfig, check, lib
+from ._lib import Situations as _Situations
 from .api import FactoryContext, _masks, _ptr
 from .gp import EvolutionConfig, TreeConfig
 from .rules import _OPCODE, Node, Packed, term_index, to_prefix
 
-__all__ = ["DeviceGP", "tokens_to_tree"]
+__all__ = ["DeviceGP", "RunResult", "tokens_to_tree", "SUB_STREAM"]
+
+SUB_STREAM = 2 ** 32   # sub-step s of a screened run draws from Mt19937(seed + s * SUB_STREAM)
 
 
 def tokens_to_tree(tokens: Sequence[int], terminals: Sequence[str], functions: Sequence[str]) -> Node:
@@ -53,13 +62,25 @@ def tokens_to_tree(tokens: Sequence[int], terminals: Sequence[str], functions: S
     return tree
 
 
+@dataclass
+class RunResult:
+    fitness: np.ndarray          # (generations, runs, pop)
+    best: List[List[Node]]       # [generation][run]: the best rule of each evaluation
+    generation_ms: np.ndarray    # device time from the start to the end of each generation
+
+    @property
+    def ms(self) -> float:
+        return float(self.generation_ms[-1])
+
+
 class DeviceGP:
     """Populations of `len(seeds)` runs on `ctx`'s instance; run r uses
     Mt19937(seeds[r]) and reads scenario rows r*T .. r*T+T-1 of each
     generation (the reference's grouped evaluation, group_size = population)."""
 
     def __init__(self, ctx: FactoryContext, tree_cfg: TreeConfig, evo_cfg: EvolutionConfig,
-                 seeds: Sequence[int], T: int, term_mask, waves_per_item: int = 0, mapping: int = MAP_AUTO):
+                 seeds: Sequence[int], T: int, term_mask, waves_per_item: int = 0, mapping: int = MAP_AUTO,
+                 offspring: int = 0):
         self.ctx = ctx
         self.tree_cfg = tree_cfg
         self.evo_cfg = evo_cfg
@@ -81,7 +102,8 @@ class DeviceGP:
         cfg = GpConfig()
         cfg.n_runs = self.runs
         cfg.population = self.pop
-        cfg.offspring = 0
+        cfg.offspring = int(offspring)
+        self.offspring = int(offspring) or self.pop
         cfg.n_terminals = len(slots)
         cfg.terminal_slots = self._slots.ctypes.data
         cfg.n_functions = len(self._fops)
@@ -113,6 +135,16 @@ class DeviceGP:
         except Exception:
             pass
 
+    def set_surrogate(self, net, sit) -> None:
+        """Screen offspring with `net` (sigp.surrogate.Mlp, kept alive here)
+        on the decision situations `sit` (64 x 8)."""
+        arrs = [np.ascontiguousarray(sit.cur, dtype=np.int32), np.ascontiguousarray(sit.ct, dtype=np.float64),
+                np.ascontiguousarray(sit.remaining, dtype=np.int32),
+                np.ascontiguousarray(sit.visited, dtype=np.uint8), np.ascontiguousarray(sit.cand, dtype=np.int32)]
+        s = _Situations(sit.n_sit, sit.n_cand, *[a.ctypes.data for a in arrs])
+        check(lib().sigp_gp_set_surrogate(self._h, net._h, C.byref(s)))
+        self._net = net
+
     def init(self) -> None:
         """ramped_half_and_half of every run (engine/gp.hpp:67-77)."""
         check(lib().sigp_gp_init(self._h))
@@ -142,16 +174,22 @@ class DeviceGP:
             raise ValueError("need %d fitness values" % (self.runs * self.pop))
         check(lib().sigp_gp_step(self._h, _ptr(f)))
 
-    def run(self, sampled_all, generations: int, graph: bool = True) -> Tuple[np.ndarray, float]:
+    def run(self, sampled_all, generations: int, graph: bool = True) -> RunResult:
         """The whole loop on the device: generation g evaluates on rows
-        g*runs*T .. and steps, except the last.  Returns the fitness history
-        (generations, runs, pop) and the device time in milliseconds."""
+        g*runs*T .. and steps, except the last."""
         rows = generations * self.runs * self.T
         a = self._rows(sampled_all, rows)
         hist = np.zeros(generations * self.runs * self.pop, dtype=np.float64)
-        ms = C.c_double(0.0)
-        check(lib().sigp_gp_run(self._h, _ptr(a), rows, generations, 1 if graph else 0, _ptr(hist), C.byref(ms)))
-        return hist.reshape(generations, self.runs, self.pop), ms.value
+        btok = np.zeros(generations * self.runs * GP_MAX_NODES, dtype=np.int16)
+        blen = np.zeros(generations * self.runs, dtype=np.int32)
+        gms = np.zeros(generations, dtype=np.float64)
+        check(lib().sigp_gp_run(self._h, _ptr(a), rows, generations, 1 if graph else 0, _ptr(hist), _ptr(btok),
+                                _ptr(blen), _ptr(gms)))
+        btok = btok.reshape(generations, self.runs, GP_MAX_NODES)
+        blen = blen.reshape(generations, self.runs)
+        best = [[tokens_to_tree(btok[g, r, :blen[g, r]], self.tree_cfg.terminals, self.tree_cfg.functions)
+                 for r in range(self.runs)] for g in range(generations)]
+        return RunResult(hist.reshape(generations, self.runs, self.pop), best, gms)
 
     def tokens(self) -> Tuple[np.ndarray, np.ndarray]:
         """(tokens (runs, pop, GP_MAX_NODES) int16, lengths (runs, pop))."""
@@ -188,8 +226,8 @@ class DeviceGP:
         return Packed(np.ascontiguousarray(np.concatenate(o_ops)), np.ascontiguousarray(np.concatenate(o_args)),
                       np.array(offs, dtype=np.int32), np.ascontiguousarray(ln))
 
-    def rng_state(self, run: int = 0) -> Tuple[int, ...]:
+    def rng_state(self, run: int = 0, sub: int = 0) -> Tuple[int, ...]:
         """As random.Random.getstate()[1]: 624 words and the position."""
         st = np.zeros(625, dtype=np.uint32)
-        check(lib().sigp_gp_rng_state(self._h, run, _ptr(st)))
+        check(lib().sigp_gp_rng_state(self._h, run, sub, _ptr(st)))
         return tuple(int(v) for v in st)

@@ -69,11 +69,14 @@ def test_device_gp_loop_reproduces_gp_fixture(ci, graph):
     gp.init()
     rows = np.stack([g["sampled"] for g in case["gens"]])
     print("gp loop", case["kind"], "graph" if graph else "stream", flush=True)
-    hist, ms = gp.run(rows, generations=3, graph=graph)
-    print("3 generations in %.2f ms" % ms, flush=True)
+    res = gp.run(rows, generations=3, graph=graph)
+    print("3 generations in %.2f ms" % res.ms, flush=True)
     for gi, g in enumerate(case["gens"]):
-        assert_bitwise(hist[gi, 0], g["fits"], "fitness of generation %d" % gi)
+        assert_bitwise(res.fitness[gi, 0], g["fits"], "fitness of generation %d" % gi)
+        # the recorded best rule: first maximum of the generation's fitness
+        assert to_prefix(res.best[gi][0]) == g["pop"][int(np.argmax(g["fits"]))]
     assert gp.prefixes(0) == case["gens"][2]["pop"]
+    assert np.all(np.diff(res.generation_ms) > 0)
 
 
 def _host_runs(tc, ec, seeds):
@@ -139,8 +142,9 @@ def test_device_gp_loop_equals_host_loop():
         gp = DeviceGP(ctx, tc, ec, seeds=seeds, T=T, term_mask=mask)
         gp.init()
         print("device loop", "graph" if graph else "stream", flush=True)
-        hist[graph], ms = gp.run(rows, generations=gens, graph=graph)
-        print("  %.1f ms" % ms, flush=True)
+        res = gp.run(rows, generations=gens, graph=graph)
+        hist[graph] = res.fitness
+        print("  %.1f ms" % res.ms, flush=True)
         pops[graph] = [[to_prefix(t) for t in p] for p in gp.population()]
     assert_bitwise(hist[True], hist[False], "graph vs stream")
     assert pops[True] == pops[False]
@@ -154,3 +158,46 @@ def test_device_gp_loop_equals_host_loop():
         if g + 1 < gens:
             hp = [G.gp_step(hp[r], list(hist[True][g, r]), rngs[r], tc, ec) for r in range(len(seeds))]
     assert pops[True] == [[to_prefix(t) for t in p] for p in hp]
+
+
+def test_device_screened_steps_match_host_restatement():
+    """Surrogate-screened steps (no reference counterpart): k = 4 gp_steps on
+    the streams Mt19937(seed + s * 2^32), phenotypes on 64 x 8 situations,
+    scores from the device network, then the elites and the best-scored
+    offspring outside the elites' slots -- against the same step restated on
+    the host with the host operators, sigp_phenotype and sigp_mlp_predict."""
+    from sigp.api import FactoryContext
+    from sigp.gp_device import SUB_STREAM, DeviceGP
+    from sigp.surrogate import Mlp, features, make_situations, phenotype
+    inst = load_instance("c101")
+    ctx = FactoryContext(inst)
+    terms = list(NS_TERMINALS) + list(EXTRA)
+    mask = term_mask_from(terms)
+    pop, k, seed = 64, 4, 7
+    tc = G.TreeConfig(terms, list(FUNCTIONS), max_depth=6)
+    ec = G.EvolutionConfig(population_size=pop, elitism=3, max_depth=7)
+    gp = DeviceGP(ctx, tc, ec, seeds=[seed], T=2, term_mask=mask, offspring=k * pop)
+    net = Mlp(seed=3)
+    sit = make_situations(inst, seed=5)
+    gp.set_surrogate(net, sit)
+    gp.init()
+    rngs = [random.Random(seed + s * SUB_STREAM) for s in range(k)]
+    host = G.ramped_half_and_half(tc, rngs[0], pop)
+    fr = np.random.default_rng(5)
+    E = min(ec.elitism, pop)
+    for gen in range(4):
+        print("screened step", gen, flush=True)
+        assert gp.prefixes(0) == [to_prefix(t) for t in host], gen
+        for s in range(k):
+            assert gp.rng_state(0, s) == _state(rngs[s]), (gen, s)
+        if gen == 3:
+            break
+        fit = fr.integers(0, 9, size=pop).astype(np.float64)
+        gp.step(fit[None, :])
+        off = []
+        for s in range(k):
+            off += G.gp_step(host, list(fit), rngs[s], tc, ec)
+        pred = net.predict(features(phenotype(ctx, pack_trees(off), mask, sit)))
+        cand = [j for j in range(k * pop) if j % pop >= E]
+        key = lambda j: (-float(pred[j]) if pred[j] == pred[j] else float("inf"), j)
+        host = off[:E] + [off[j] for j in sorted(cand, key=key)[:pop - E]]
