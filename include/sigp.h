@@ -303,8 +303,10 @@ sigp_status sigp_gp_step(sigp_gp *gp, const double *fitness);
  * offspring are compiled on the device, their phenotypes taken on `sit`
  * (n_sit x n_cand = SIGP_MLP_IN, as sigp_phenotype) and scored by `net`
  * (not owned; it must outlive the run); the next parents are the elites
- * and then the population - elitism best-scored offspring outside the
- * elites' slots, best first, equal scores by slot. */
+ * and then population - elitism offspring outside the elites' slots:
+ * phenotypes not seen at an earlier slot of the run first (phenotype =
+ * FNV-1a 64-bit hash of the ranks, one byte each), then higher score, then
+ * lower slot. */
 sigp_status sigp_gp_set_surrogate(sigp_gp *gp, sigp_mlp *net, const sigp_situations *sit);
 /* The whole loop on the device: for g < generations, evaluate on rows
  * g*n_runs*T .. (n_rows = generations*n_runs*T) and step, except after the

@@ -468,34 +468,74 @@ __global__ __launch_bounds__(64) void gather_kernel(GpParams P, const int32_t *s
     if (threadIdx.x == 0) plen[dst] = L;
 }
 
+// Phenotype hash of each offspring: FNV-1a (64-bit) over its ranks, one
+// byte per rank in situation-major order.  One wavefront per offspring.
+__global__ __launch_bounds__(64) void phash_kernel(const int32_t *ranks, int n_feat, uint64_t *hash) {
+    const size_t q = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (int k = 0; k < n_feat; ++k) {
+        h ^= (uint64_t)(uint8_t)ranks[q * n_feat + k];
+        h *= 0x100000001b3ull;
+    }
+    hash[q] = h;
+}
+
+// An offspring is a phenotypic duplicate when an earlier slot of its run
+// (any sub-step, elites included) has the same phenotype hash.
+__global__ __launch_bounds__(256) void dup_kernel(GpParams P, const uint64_t *hash, uint8_t *dup) {
+    __shared__ uint64_t tile[1024];
+    const int r = (int)blockIdx.y;
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const uint64_t *hr = hash + (size_t)r * P.n_off;
+    const uint64_t mine = j < P.n_off ? hr[j] : 0;
+    const int last = min(P.n_off, (int)((blockIdx.x + 1) * blockDim.x));   // slots before this block's last
+    bool d = false;
+    for (int b = 0; b < last; b += 1024) {
+        for (int q = (int)threadIdx.x; q < 1024; q += (int)blockDim.x) tile[q] = b + q < P.n_off ? hr[b + q] : 0;
+        __syncthreads();
+        const int m = min(1024, j - b);
+        for (int q = 0; q < m; ++q) d |= tile[q] == mine;
+        __syncthreads();
+    }
+    if (j < P.n_off) dup[(size_t)r * P.n_off + j] = d ? 1 : 0;
+}
+
 // The next parents of a screened step: run r keeps its E elites (the first
-// slots of sub-step 0), then the pop - E offspring with the highest
-// predicted reward among the non-elite slots of all sub-steps, best first,
-// equal predictions by slot; a NaN prediction ranks as -infinity.  One
-// thread per slot counts the candidates ranked before it.
-__global__ __launch_bounds__(256) void select_kernel(GpParams P, const float *pred, int32_t *sel) {
-    __shared__ float tile[1024];
+// slots of sub-step 0), then pop - E of the offspring outside the elites'
+// slots, in the order: phenotypes not seen at an earlier slot first, then
+// higher predicted reward (NaN as -infinity), then lower slot.  One thread
+// per slot counts the candidates ordered before it.
+__global__ __launch_bounds__(256) void select_kernel(GpParams P, const float *pred, const uint8_t *dup,
+                                                     int32_t *sel) {
+    __shared__ float tv[1024];
+    __shared__ uint8_t td[1024];
     const int r = (int)blockIdx.y;
     const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int E = min(P.elitism, P.pop), K = P.pop - E;
     const float *pr = pred + (size_t)r * P.n_off;
-    // candidates' keys; NaN marks a non-candidate (it compares false)
+    const uint8_t *dr = dup + (size_t)r * P.n_off;
+    // td: 0 distinct, 1 duplicate, 2 not a candidate (an elite's slot)
+    auto cls = [&](int q) { return q % P.pop < E ? (uint8_t)2 : dr[q]; };
     auto key = [&](int q) {
-        if (q % P.pop < E) return __builtin_nanf("");
         const float v = pr[q];
         return v != v ? -INFINITY : v;
     };
     const bool cand = j < P.n_off && j % P.pop >= E;
-    const float mine = cand ? key(j) : 0.0f;
+    const uint8_t mc = cand ? cls(j) : 2;
+    const float mv = cand ? key(j) : 0.0f;
     int rank = 0;
     for (int b = 0; b < P.n_off; b += 1024) {
-        for (int q = (int)threadIdx.x; q < 1024; q += (int)blockDim.x)
-            tile[q] = b + q < P.n_off ? key(b + q) : __builtin_nanf("");
+        for (int q = (int)threadIdx.x; q < 1024; q += (int)blockDim.x) {
+            td[q] = b + q < P.n_off ? cls(b + q) : 2;
+            tv[q] = b + q < P.n_off ? key(b + q) : 0.0f;
+        }
         __syncthreads();
         const int m = min(1024, P.n_off - b);
         for (int q = 0; q < m; ++q) {
-            const float v = tile[q];
-            rank += (v > mine || (v == mine && b + q < j)) ? 1 : 0;
+            const uint8_t c = td[q];
+            const float v = tv[q];
+            rank += (c < mc || (c == mc && (v > mv || (v == mv && b + q < j)))) ? 1 : 0;
         }
         __syncthreads();
     }
@@ -768,6 +808,8 @@ struct sigp_gp {
     double *fit = nullptr, *pairs = nullptr, *scen = nullptr;
     int *gen = nullptr;
     int32_t *ranks = nullptr, *sel = nullptr;   // screened steps: phenotypes, next parents
+    uint64_t *phash = nullptr;                  // screened steps: phenotype hashes
+    uint8_t *dup = nullptr;                     // screened steps: phenotypic duplicates
     float *pred = nullptr;                      // screened steps: predicted rewards
     sigp_batch *bt = nullptr;         // the parents' evaluation
     sigp_batch *obt = nullptr;        // the offspring's phenotype
@@ -809,7 +851,9 @@ static sigp_status launch_step(sigp_gp *g) {
         st = sigp_internal_mlp_predict_ranks(g->net, g->ranks, (int64_t)g->runs * p.n_off, g->sit.n_cand, g->pred,
                                              g->stream);
         if (st != SIGP_OK) return st;
-        select_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->pred, g->sel);
+        phash_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(g->ranks, SIGP_MLP_IN, g->phash);
+        dup_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->phash, g->dup);
+        select_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->pred, g->dup, g->sel);
         sel = g->sel;
     }
     gather_kernel<<<dim3(p.pop, g->runs), 64, 0, g->stream>>>(p, sel, g->otok, g->olen, g->ptok, g->plen);
@@ -927,7 +971,8 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
                  o_pairs = take(P * (size_t)g->T * 8), o_scen = take((size_t)g->runs * g->T * g->n * 8),
                  o_gen = take(4), o_oops = take(OCL * 4), o_oargs = take(OCL * 8), o_ooff = take(OC * 4),
                  o_olen2 = take(OC * 4), o_ord2 = take(OC * 8), o_ofl = take(OC * 8), o_ocost = take(OC * 4),
-                 o_ranks = take(OC * SIGP_MLP_IN * 4), o_pred = take(OC * 4), o_sel = take(screened ? P * 4 : 0);
+                 o_ranks = take(OC * SIGP_MLP_IN * 4), o_pred = take(OC * 4), o_sel = take(screened ? P * 4 : 0),
+                 o_hash = take(OC * 8), o_dup = take(OC);
     if (hipMalloc((void **)&g->dbuf, off) != hipSuccess) {
         delete g;
         return sigp_internal_fail(SIGP_OOM, "GP allocation failed");
@@ -972,6 +1017,8 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     g->ranks = (int32_t *)(b + o_ranks);
     g->pred = (float *)(b + o_pred);
     g->sel = (int32_t *)(b + o_sel);
+    g->phash = (uint64_t *)(b + o_hash);
+    g->dup = (uint8_t *)(b + o_dup);
 
     // run r's sub-step s draws from Mt19937(seeds[r] + s * 2^32)
     std::vector<uint32_t> st((size_t)g->runs * p.subs * MT_WORDS);

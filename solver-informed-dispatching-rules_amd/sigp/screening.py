@@ -1,21 +1,23 @@
 """Surrogate pre-screening experiments (no counterpart in the reference).
 
-    python -m sigp.screening --rules 100000 --generations 20
+    python -m sigp.screening --rules 100000 --generations 50
 
 1. Labels: `--rules` random rules (ramped half-and-half over NS-11 +
-   EXTRA-18) simulated exactly over 50 training scenarios; their
-   phenotypes (64 situations x 8 candidates) computed on the device.
+   EXTRA-18) simulated exactly over 50 label scenarios; their phenotypes
+   (64 situations x 8 candidates) computed on the device.
 2. Network: the 512-256-256-1 MLP trained on 80 % of them; Spearman rank
    correlation and mean absolute error against the exact rewards on the
    other 20 %.
 3. Screening: from a pool of fresh rules, the network's best eighth against
    a random eighth, both scored exactly -- what the network adds over
    picking without it.
-4. GP: plain GP (each generation's 1,024 offspring simulated exactly) against
-   surrogate-assisted GP (8,192 offspring bred, the network keeps the best
-   1,024 for exact simulation), same instance, seeds and scenario schedule;
-   held-out reward (500 scenarios) of each generation's best rule, and
-   wall time split into host GP, exact simulation and the network.
+4. GP, both loops entirely on the device: plain GP (each generation's 1,024
+   offspring simulated exactly on 50 training scenarios) against
+   surrogate-assisted GP (8 x 1,024 offspring bred, the network keeps the
+   best 1,024 for exact simulation), same instance, seeds and scenario
+   schedule.  Each generation's best rule (by its training fitness) is
+   scored on 500 held-out scenarios; the result is the device time at which
+   each loop first reaches plain GP's final held-out reward within 0.5 %.
 
 Prints one JSON object.  Every reward is computed by the exact simulator.
 """
@@ -31,7 +33,7 @@ from typing import List
 import numpy as np
 
 from .api import FactoryContext
-from .gp import EvolutionConfig, TreeConfig, gp_step, ramped_half_and_half
+from .gp import EvolutionConfig, TreeConfig, ramped_half_and_half
 from .instance import Instance, synthetic_clustered
 from .rules import EXTRA, FUNCTIONS, NS_TERMINALS, Node, pack_trees, term_mask_from
 from .scenarios import gen_durations
@@ -116,40 +118,38 @@ def screening_gain(sc: Scorer, net: Mlp, norm, pool: int, seed: int):
             "spearman_pool": round(spearman(pred, true), 4)}
 
 
-def run_gp(sc: Scorer, net, norm, generations: int, pop: int, breed: int, seed: int, label: str):
-    """One GP run; with net, each generation breeds `breed` offspring and the
-    network keeps the best `pop` for exact simulation."""
+def run_device_gp(sc: Scorer, net, generations: int, pop: int, breed: int, seed: int, label: str):
+    """One GP loop on the device (one HIP graph per generation): plain, or
+    screened by `net` when given (breed = k x pop offspring per generation).
+    Returns the held-out reward of each generation's best rule and the device
+    time at the end of each generation."""
+    from .gp_device import DeviceGP
     tc = TreeConfig(list(VOCAB), list(FUNCTIONS), 8, 2)
-    ec = EvolutionConfig()
-    rng = random.Random(seed)
-    t_host = 0.0
-    t0 = time.perf_counter()
-    popl = ramped_half_and_half(tc, rng, pop)
-    t_host += time.perf_counter() - t0
-    curve = []
-    wall0 = time.perf_counter()
-    for gen in range(generations):
-        fit = sc.exact(popl, sc.train_scenarios(gen)).tolist()
-        best = popl[int(np.argmax(fit))]
-        held = float(sc.exact([best], sc.test)[0])
-        curve.append(round(held, 3))
-        _log("%s gen %d best held-out %.2f" % (label, gen, held))
-        if gen == generations - 1:
-            break
-        t0 = time.perf_counter()
-        if net is None:
-            popl = gp_step(popl, fit, rng, tc, ec)
-            t_host += time.perf_counter() - t0
-        else:
-            pool: List[Node] = []
-            while len(pool) < breed:
-                pool.extend(gp_step(popl, fit, rng, tc, ec))
-            t_host += time.perf_counter() - t0
-            pred = net.predict(sc.features(pool)) * norm[1] + norm[0]
-            keep = np.argsort(-pred, kind="stable")[:pop]
-            popl = [pool[int(i)] for i in keep]
-    return {"held_out_best_per_generation": curve, "wall_s": round(time.perf_counter() - wall0, 2),
-            "host_gp_s": round(t_host, 2)}
+    ec = EvolutionConfig(population_size=pop)
+    T = sc.n_train
+    rows = np.concatenate([sc.train_scenarios(g) for g in range(generations)])
+    gp = DeviceGP(sc.ctx, tc, ec, seeds=[seed], T=T, term_mask=sc.mask,
+                  offspring=breed if net is not None else 0)
+    if net is not None:
+        gp.set_surrogate(net, sc.sit)
+    gp.init()
+    res = gp.run(rows, generations=generations, graph=True)
+    best = [res.best[g][0] for g in range(generations)]
+    held = sc.exact(best, sc.test)
+    _log("%s: %d generations in %.1f ms; held-out of the last best %.2f"
+         % (label, generations, res.ms, float(held[-1])))
+    gp.close()
+    return {"held_out_best_per_generation": [round(float(v), 3) for v in held],
+            "generation_ms": [round(float(v), 1) for v in res.generation_ms]}
+
+
+def reach(curve, ms, target):
+    """First generation whose held-out reward is within 0.5 % of `target`,
+    and the device time at its end."""
+    for g, v in enumerate(curve):
+        if v >= 0.995 * target:
+            return g, ms[g]
+    return None, None
 
 
 def main(argv=None):
@@ -157,10 +157,11 @@ def main(argv=None):
     ap.add_argument("--rules", type=int, default=100_000)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--pool", type=int, default=8192)
-    ap.add_argument("--generations", type=int, default=20)
+    ap.add_argument("--generations", type=int, default=50)
     ap.add_argument("--pop", type=int, default=1024)
     ap.add_argument("--breed", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--gp-runs", type=int, default=5, help="GP seed pairs (plain, surrogate) with one network")
     a = ap.parse_args(argv)
     inst = synthetic_clustered(n=100, m=3, seed=1)
     sc = Scorer(inst)
@@ -171,16 +172,29 @@ def main(argv=None):
     out = {"instance": "clustered n=100 m=3 (windows U[30,120])", "vocabulary": "NS-11 + EXTRA-18",
            "train_scenarios": 50, "held_out_scenarios": 500, "network": stats, "screening": gain}
     if a.generations > 0:
-        sc.t_exact = sc.t_net = 0.0
-        plain = run_gp(sc, None, norm, a.generations, a.pop, a.pop, a.seed + 7, "plain")
-        plain.update(exact_s=round(sc.t_exact, 2))
-        sc.t_exact = sc.t_net = 0.0
-        surr = run_gp(sc, net, norm, a.generations, a.pop, a.breed, a.seed + 7, "surrogate")
-        surr.update(exact_s=round(sc.t_exact, 2), network_s=round(sc.t_net, 2))
-        target = plain["held_out_best_per_generation"][-1]
-        reach = next((g for g, v in enumerate(surr["held_out_best_per_generation"]) if v >= 0.995 * target), None)
-        out["gp"] = {"generations": a.generations, "pop": a.pop, "breed": a.breed, "plain": plain,
-                     "surrogate": surr, "surrogate_generation_reaching_plain_final_within_0.5pct": reach}
+        # warm-up of the device loops (code objects, graph instantiation)
+        run_device_gp(sc, None, 2, 64, 64, a.seed + 7, "warm-up")
+        run_device_gp(sc, net, 2, 64, 512, a.seed + 7, "warm-up")
+        setup_s = sum(stats["seconds"].values())
+        runs = []
+        for i in range(a.gp_runs):
+            gseed = a.seed + 7 + i
+            plain = run_device_gp(sc, None, a.generations, a.pop, a.pop, gseed, "plain seed %d" % gseed)
+            surr = run_device_gp(sc, net, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed)
+            target = plain["held_out_best_per_generation"][-1]
+            pg, pms = reach(plain["held_out_best_per_generation"], plain["generation_ms"], target)
+            sgen, sms = reach(surr["held_out_best_per_generation"], surr["generation_ms"], target)
+            total = plain["generation_ms"][-1]
+            runs.append({"seed": gseed, "plain": plain, "surrogate": surr, "target_held_out": target,
+                         "plain_reaches_target": {"generation": pg, "device_ms": pms},
+                         "surrogate_reaches_target": {"generation": sgen, "device_ms": sms},
+                         "time_ratio": None if sms is None else round(sms / total, 4)})
+            _log("seed %d: target %.2f, surrogate reaches it at generation %s (%.0f of %.0f ms)"
+                 % (gseed, target, sgen, sms if sms is not None else float("nan"), total))
+        ratios = [r["time_ratio"] if r["time_ratio"] is not None else float("inf") for r in runs]
+        out["gp"] = {"generations": a.generations, "pop": a.pop, "breed": a.breed, "runs": runs,
+                     "time_ratios": ratios, "median_time_ratio": float(np.median(ratios)),
+                     "network_setup_s": round(setup_s, 3)}
     print(json.dumps(out), flush=True)
 
 

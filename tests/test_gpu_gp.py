@@ -164,8 +164,9 @@ def test_device_screened_steps_match_host_restatement():
     """Surrogate-screened steps (no reference counterpart): k = 4 gp_steps on
     the streams Mt19937(seed + s * 2^32), phenotypes on 64 x 8 situations,
     scores from the device network, then the elites and the best-scored
-    offspring outside the elites' slots -- against the same step restated on
-    the host with the host operators, sigp_phenotype and sigp_mlp_predict."""
+    offspring outside the elites' slots, phenotypic duplicates last -- against
+    the same step restated on the host with the host operators,
+    sigp_phenotype and sigp_mlp_predict."""
     from sigp.api import FactoryContext
     from sigp.gp_device import SUB_STREAM, DeviceGP
     from sigp.surrogate import Mlp, features, make_situations, phenotype
@@ -197,7 +198,21 @@ def test_device_screened_steps_match_host_restatement():
         off = []
         for s in range(k):
             off += G.gp_step(host, list(fit), rngs[s], tc, ec)
-        pred = net.predict(features(phenotype(ctx, pack_trees(off), mask, sit)))
+        ranks = phenotype(ctx, pack_trees(off), mask, sit)
+        pred = net.predict(features(ranks))
+        seen, dup = set(), []
+        for j in range(k * pop):
+            h = _fnv1a(ranks[j])
+            dup.append(h in seen)
+            seen.add(h)
         cand = [j for j in range(k * pop) if j % pop >= E]
-        key = lambda j: (-float(pred[j]) if pred[j] == pred[j] else float("inf"), j)
+        key = lambda j: (dup[j], -float(pred[j]) if pred[j] == pred[j] else float("inf"), j)
         host = off[:E] + [off[j] for j in sorted(cand, key=key)[:pop - E]]
+        print("  distinct phenotypes among %d offspring: %d" % (k * pop, k * pop - sum(dup)), flush=True)
+
+
+def _fnv1a(ranks):
+    h = 0xCBF29CE484222325
+    for v in np.asarray(ranks).ravel():
+        h = ((h ^ (int(v) & 0xFF)) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
