@@ -310,14 +310,18 @@ sigp_status sigp_gp_step(sigp_gp *gp, const double *fitness);
 sigp_status sigp_gp_set_surrogate(sigp_gp *gp, sigp_mlp *net, const sigp_situations *sit);
 /* The whole loop on the device: for g < generations, evaluate on rows
  * g*n_runs*T .. (n_rows = generations*n_runs*T) and step, except after the
- * last.  use_graph != 0 captures one generation as a HIP graph and replays
- * it.  Outputs (each nullable): fitness_history, generations x n_runs x
+ * last unless flags has SIGP_GP_STEP_LAST (a loop run in several calls).
+ * SIGP_GP_GRAPH captures one generation as a HIP graph and replays it.
+ * Outputs (each nullable), numbered from this call's first generation:
+ * fitness_history, generations x n_runs x
  * population; best_tokens / best_lengths, each generation's and run's best
  * rule (first maximum of the fitness), generations x n_runs x
  * SIGP_GP_MAX_NODES tokens and lengths; generation_ms, the device time from
  * the loop's start to the end of each generation. */
+#define SIGP_GP_GRAPH 1
+#define SIGP_GP_STEP_LAST 2
 sigp_status sigp_gp_run(sigp_gp *gp, const double *sampled, int64_t n_rows, int32_t generations,
-                        int32_t use_graph, double *fitness_history, int16_t *best_tokens,
+                        int32_t flags, double *fitness_history, int16_t *best_tokens,
                         int32_t *best_lengths, double *generation_ms);
 /* The population: tokens (n_runs*population*SIGP_GP_MAX_NODES) and lengths. */
 sigp_status sigp_gp_population(const sigp_gp *gp, int16_t *tokens, int32_t *lengths);

@@ -1157,8 +1157,9 @@ static sigp_status launch_generation(sigp_gp *g, const double *all, double *hist
 }
 
 extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_rows, int32_t generations,
-                                   int32_t use_graph, double *fitness_history, int16_t *best_tokens,
+                                   int32_t flags, double *fitness_history, int16_t *best_tokens,
                                    int32_t *best_lengths, double *generation_ms) {
+    const bool use_graph = (flags & SIGP_GP_GRAPH) != 0, step_last = (flags & SIGP_GP_STEP_LAST) != 0;
     if (!g || !sampled) return gp_fail("NULL argument");
     sigp_status st0 = check_step(g);
     if (st0 != SIGP_OK) return st0;
@@ -1188,7 +1189,7 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
         if (!check(hipEventCreate(&e), "event")) break;
     if (st == SIGP_OK && check(hipMemcpyAsync(all, sampled, sbytes, hipMemcpyHostToDevice, g->stream), "upload")) {
         set_kernel<<<1, 64, 0, g->stream>>>(g->gen, 0);
-        if (use_graph && generations > 1) {
+        if (use_graph && (generations > 1 || step_last)) {
             // capture one stepping generation; the counter it reads advances on the device
             if (check(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "begin capture")) {
                 const sigp_status s2 = launch_generation(g, all, hist, btok, blen, true);
@@ -1199,7 +1200,7 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
         }
         if (st == SIGP_OK) check(hipEventRecord(ev[0], g->stream), "event");
         for (int gi = 0; st == SIGP_OK && gi < generations; ++gi) {
-            const bool step = gi + 1 < generations;
+            const bool step = gi + 1 < generations || step_last;
             if (exec && step) check(hipGraphLaunch(exec, g->stream), "graph launch");
             else {
                 const sigp_status s2 = launch_generation(g, all, hist, btok, blen, step);

@@ -1314,19 +1314,13 @@ static int resolved_mapping(const sigp_batch *bt, const LaunchUnit &u) {
 }
 
 // Wavefronts per workgroup: the tuned or configured count, else 4 for
-// scenario lanes; for candidate lanes one wavefront up to 128 customers (two
-// passes of 64 candidates measured faster than two half-idle wavefronts at
-// n = 100), else enough 64-candidate groups to cover the class's largest
-// instance (at most 4); then reduced until LDS fits.
+// scenario lanes and 1 for candidate lanes (one wavefront making several
+// passes over the candidates measured fastest: at n = 100 and at n = 288,
+// 1,024 / 4,096 rules x 50 scenarios, 1 wavefront beat 2 and 4 by 15-60 %);
+// then reduced until LDS fits.
 static int resolved_waves(const sigp_batch *bt, const LaunchUnit &u, int mapping) {
     int w = u.waves ? u.waves : bt->waves_per_item;
-    if (w == 0) {
-        w = 4;
-        if (mapping == SIGP_MAP_CANDIDATES) {
-            const int need = (u.max_n + 63) / 64;
-            w = u.max_n <= 128 ? 1 : (need <= 2 ? 2 : 4);
-        }
-    }
+    if (w == 0) w = mapping == SIGP_MAP_CANDIDATES ? 1 : 4;
     return fit_waves(bt, w);
 }
 

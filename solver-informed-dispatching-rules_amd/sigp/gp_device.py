@@ -174,16 +174,18 @@ class DeviceGP:
             raise ValueError("need %d fitness values" % (self.runs * self.pop))
         check(lib().sigp_gp_step(self._h, _ptr(f)))
 
-    def run(self, sampled_all, generations: int, graph: bool = True) -> RunResult:
+    def run(self, sampled_all, generations: int, graph: bool = True, step_last: bool = False) -> RunResult:
         """The whole loop on the device: generation g evaluates on rows
-        g*runs*T .. and steps, except the last."""
+        g*runs*T .. and steps, except the last unless `step_last` (a loop
+        run in several calls).  Generations are numbered from this call."""
         rows = generations * self.runs * self.T
         a = self._rows(sampled_all, rows)
         hist = np.zeros(generations * self.runs * self.pop, dtype=np.float64)
         btok = np.zeros(generations * self.runs * GP_MAX_NODES, dtype=np.int16)
         blen = np.zeros(generations * self.runs, dtype=np.int32)
         gms = np.zeros(generations, dtype=np.float64)
-        check(lib().sigp_gp_run(self._h, _ptr(a), rows, generations, 1 if graph else 0, _ptr(hist), _ptr(btok),
+        flags = (1 if graph else 0) | (2 if step_last else 0)
+        check(lib().sigp_gp_run(self._h, _ptr(a), rows, generations, flags, _ptr(hist), _ptr(btok),
                                 _ptr(blen), _ptr(gms)))
         btok = btok.reshape(generations, self.runs, GP_MAX_NODES)
         blen = blen.reshape(generations, self.runs)

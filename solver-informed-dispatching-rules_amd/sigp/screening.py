@@ -34,7 +34,7 @@ import numpy as np
 
 from .api import FactoryContext
 from .gp import EvolutionConfig, TreeConfig, ramped_half_and_half
-from .instance import Instance, synthetic_clustered
+from .instance import Instance, synthetic_clustered, synthetic_mixed
 from .rules import EXTRA, FUNCTIONS, NS_TERMINALS, Node, pack_trees, term_mask_from
 from .scenarios import gen_durations
 from .surrogate import Mlp, features, make_situations, phenotype, spearman
@@ -50,13 +50,15 @@ def _log(*a):
 class Scorer:
     """Exact mean rewards and surrogate scores of rule lists on one instance."""
 
-    def __init__(self, inst: Instance, n_train: int = 50):
+    def __init__(self, inst: Instance, n_train: int = 50, shape: float = 4.0, depth: int = 8):
         self.inst = inst
         self.ctx = FactoryContext(inst)
         self.mask = term_mask_from(VOCAB)
         self.sit = make_situations(inst)
         self.n_train = n_train
-        self.test = gen_durations(inst.durations, 500, TEST_SEED, mode="gamma")
+        self.shape = shape
+        self.depth = depth          # TreeConfig / EvolutionConfig max_depth
+        self.test = gen_durations(inst.durations, 500, TEST_SEED, mode="gamma", shape=shape)
         self.t_exact = 0.0
         self.t_net = 0.0
 
@@ -68,7 +70,8 @@ class Scorer:
         return mean
 
     def train_scenarios(self, gen: int) -> np.ndarray:
-        return gen_durations(self.inst.durations, self.n_train, 2_000_000 + 1000 * gen, mode="gamma")
+        return gen_durations(self.inst.durations, self.n_train, 2_000_000 + 1000 * gen, mode="gamma",
+                             shape=self.shape)
 
     def features(self, trees: List[Node]) -> np.ndarray:
         t0 = time.perf_counter()
@@ -78,7 +81,7 @@ class Scorer:
 
 
 def train_network(sc: Scorer, n_rules: int, epochs: int, seed: int):
-    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), 8, 2)
+    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
     trees = ramped_half_and_half(tc, random.Random(seed), n_rules)
     scen = sc.train_scenarios(0)
     t0 = time.perf_counter()
@@ -105,7 +108,7 @@ def train_network(sc: Scorer, n_rules: int, epochs: int, seed: int):
 
 
 def screening_gain(sc: Scorer, net: Mlp, norm, pool: int, seed: int):
-    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), 8, 2)
+    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
     trees = ramped_half_and_half(tc, random.Random(seed), pool)
     pred = net.predict(sc.features(trees)) * norm[1] + norm[0]
     true = sc.exact(trees, sc.train_scenarios(1))
@@ -118,29 +121,38 @@ def screening_gain(sc: Scorer, net: Mlp, norm, pool: int, seed: int):
             "spearman_pool": round(spearman(pred, true), 4)}
 
 
-def run_device_gp(sc: Scorer, net, generations: int, pop: int, breed: int, seed: int, label: str):
+def run_device_gp(sc: Scorer, net, generations: int, pop: int, breed: int, seed: int, label: str,
+                  chunk: int = 5, launch=None):
     """One GP loop on the device (one HIP graph per generation): plain, or
-    screened by `net` when given (breed = k x pop offspring per generation).
-    Returns the held-out reward of each generation's best rule and the device
-    time at the end of each generation."""
+    screened by `net` when given (breed = k x pop offspring per generation),
+    run in calls of `chunk` generations.  Returns the held-out reward of each
+    generation's best rule and the device time at the end of each
+    generation."""
     from .gp_device import DeviceGP
-    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), 8, 2)
-    ec = EvolutionConfig(population_size=pop)
+    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
+    ec = EvolutionConfig(population_size=pop, max_depth=sc.depth)
     T = sc.n_train
-    rows = np.concatenate([sc.train_scenarios(g) for g in range(generations)])
     gp = DeviceGP(sc.ctx, tc, ec, seeds=[seed], T=T, term_mask=sc.mask,
-                  offspring=breed if net is not None else 0)
+                  offspring=breed if net is not None else 0, **(launch or {}))
     if net is not None:
         gp.set_surrogate(net, sc.sit)
     gp.init()
-    res = gp.run(rows, generations=generations, graph=True)
-    best = [res.best[g][0] for g in range(generations)]
+    best, gms, t_dev = [], [], 0.0
+    for g0 in range(0, generations, chunk):
+        k = min(chunk, generations - g0)
+        rows = np.concatenate([sc.train_scenarios(g) for g in range(g0, g0 + k)])
+        res = gp.run(rows, generations=k, graph=True, step_last=g0 + k < generations)
+        best += [res.best[g][0] for g in range(k)]
+        gms += [t_dev + float(v) for v in res.generation_ms]
+        t_dev = gms[-1]
+        if generations > chunk:
+            _log("  %s: generation %d done, %.0f ms device time" % (label, g0 + k - 1, t_dev))
     held = sc.exact(best, sc.test)
     _log("%s: %d generations in %.1f ms; held-out of the last best %.2f"
-         % (label, generations, res.ms, float(held[-1])))
+         % (label, generations, t_dev, float(held[-1])))
     gp.close()
     return {"held_out_best_per_generation": [round(float(v), 3) for v in held],
-            "generation_ms": [round(float(v), 1) for v in res.generation_ms]}
+            "generation_ms": [round(float(v), 1) for v in gms]}
 
 
 def reach(curve, ms, target):
@@ -158,18 +170,34 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--pool", type=int, default=8192)
     ap.add_argument("--generations", type=int, default=50)
-    ap.add_argument("--pop", type=int, default=1024)
-    ap.add_argument("--breed", type=int, default=8192)
+    ap.add_argument("--config", choices=["clustered", "large"], default="clustered")
+    ap.add_argument("--waves", type=int, default=0, help="evaluation wavefronts per workgroup (0: automatic)")
+    ap.add_argument("--mapping", choices=["scenarios", "candidates"], default="candidates")
+    ap.add_argument("--pop", type=int, default=0, help="population (0: 1,024; large: 4,096)")
+    ap.add_argument("--breed", type=int, default=0, help="offspring bred per generation (0: 8 x population)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--gp-runs", type=int, default=5, help="GP seed pairs (plain, surrogate) with one network")
     a = ap.parse_args(argv)
-    inst = synthetic_clustered(n=100, m=3, seed=1)
-    sc = Scorer(inst)
+    if a.config == "large":
+        # n=288 clustered + uniform, m=4, windows U[20,200], Gamma(2), rules <= 127 nodes
+        inst = synthetic_mixed(n=288, m=4, seed=1)
+        sc = Scorer(inst, shape=2.0, depth=6)
+        desc = "clustered + uniform n=288 m=4 (windows U[20,200]), Gamma(2)"
+        a.pop = a.pop or 4096
+    else:
+        inst = synthetic_clustered(n=100, m=3, seed=1)
+        sc = Scorer(inst)
+        desc = "clustered n=100 m=3 (windows U[30,120]), Gamma(4)"
+        a.pop = a.pop or 1024
+    a.breed = a.breed or 8 * a.pop
+    launch = None
+    if a.waves:
+        launch = dict(waves_per_item=a.waves, mapping=1 if a.mapping == "scenarios" else 2)
     net, norm, stats = train_network(sc, a.rules, a.epochs, a.seed)
     _log("network", json.dumps(stats))
     gain = screening_gain(sc, net, norm, a.pool, a.seed + 100)
     _log("screening", json.dumps(gain))
-    out = {"instance": "clustered n=100 m=3 (windows U[30,120])", "vocabulary": "NS-11 + EXTRA-18",
+    out = {"instance": desc, "vocabulary": "NS-11 + EXTRA-18", "max_depth": sc.depth,
            "train_scenarios": 50, "held_out_scenarios": 500, "network": stats, "screening": gain}
     if a.generations > 0:
         # warm-up of the device loops (code objects, graph instantiation)
@@ -179,8 +207,10 @@ def main(argv=None):
         runs = []
         for i in range(a.gp_runs):
             gseed = a.seed + 7 + i
-            plain = run_device_gp(sc, None, a.generations, a.pop, a.pop, gseed, "plain seed %d" % gseed)
-            surr = run_device_gp(sc, net, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed)
+            plain = run_device_gp(sc, None, a.generations, a.pop, a.pop, gseed, "plain seed %d" % gseed,
+                                  launch=launch)
+            surr = run_device_gp(sc, net, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed,
+                                 launch=launch)
             target = plain["held_out_best_per_generation"][-1]
             pg, pms = reach(plain["held_out_best_per_generation"], plain["generation_ms"], target)
             sgen, sms = reach(surr["held_out_best_per_generation"], surr["generation_ms"], target)

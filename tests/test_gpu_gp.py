@@ -216,3 +216,27 @@ def _fnv1a(ranks):
     for v in np.asarray(ranks).ravel():
         h = ((h ^ (int(v) & 0xFF)) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
     return h
+
+
+def test_device_gp_loop_in_chunks_equals_one_call():
+    """A loop run as 2 + 3 generations (stepping after the first call's last
+    generation) equals the same 5 generations in one call."""
+    from sigp.api import FactoryContext
+    from sigp.gp_device import DeviceGP
+    inst = synthetic_clustered(n=100, m=3, seed=1)
+    ctx = FactoryContext(inst)
+    terms = list(NS_TERMINALS) + list(EXTRA)
+    mask = term_mask_from(terms)
+    tc, ec = G.TreeConfig(terms, list(FUNCTIONS)), G.EvolutionConfig(population_size=128)
+    T, gens = 3, 5
+    rows = np.concatenate([gen_durations(inst.durations, T, 6 * 10000 + g, mode="gamma") for g in range(gens)])
+    one = DeviceGP(ctx, tc, ec, seeds=[6], T=T, term_mask=mask)
+    one.init()
+    full = one.run(rows, generations=gens)
+    two = DeviceGP(ctx, tc, ec, seeds=[6], T=T, term_mask=mask)
+    two.init()
+    a = two.run(rows[:2 * T], generations=2, step_last=True)
+    b = two.run(rows[2 * T:], generations=3)
+    assert_bitwise(np.concatenate([a.fitness, b.fitness]), full.fitness, "chunked vs one call")
+    assert [to_prefix(t[0]) for t in a.best + b.best] == [to_prefix(t[0]) for t in full.best]
+    assert two.prefixes(0) == one.prefixes(0)
