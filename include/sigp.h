@@ -244,6 +244,31 @@ sigp_status sigp_mlp_train(sigp_mlp *net, const float *x, const float *t, int64_
 sigp_status sigp_mlp_predict(sigp_mlp *net, const float *x, int64_t rows, float *y_out);
 
 /* ------------------------------------------------------------------------
+ * Scenario rows on the device (engine/sim_host.hpp:43-58 gen_durations,
+ * whose rows the reference samples on the host).  Row r draws from
+ * Mt19937(base_seed + r): customer i gets max(0, gauss(mu_i, sigma mu_i))
+ * (engine/rng.hpp:135-147) or, in the Gamma mode (no counterpart in the
+ * reference), Gamma(shape, mu_i / shape) by Marsaglia & Tsang (shape >= 1);
+ * customers with mu_i <= 0 draw nothing.  The uniform stream is the
+ * reference's bit for bit; log/sin/cos are correctly rounded, where glibc's
+ * (the reference's) are not in about 0.1 % of calls, so a duration can
+ * differ from the reference's host sampler in the last place. */
+#define SIGP_SAMPLE_GAUSS 0
+#define SIGP_SAMPLE_GAMMA 1
+typedef struct {
+    int32_t mode;     /* SIGP_SAMPLE_GAUSS or SIGP_SAMPLE_GAMMA */
+    double sigma;     /* Gauss: sd = sigma * expected (the reference's 0.2) */
+    double shape;     /* Gamma: shape, scale = expected / shape */
+} sigp_sampler;
+
+/* n_rows x n durations into host memory. */
+sigp_status sigp_sample_durations(sigp_instance *inst, const sigp_sampler *smp, int64_t base_seed,
+                                  int64_t n_rows, double *out);
+/* The same into device memory on `stream` (hipStream_t), without waiting. */
+sigp_status sigp_sample_durations_async(sigp_instance *inst, const sigp_sampler *smp, int64_t base_seed,
+                                        int64_t n_rows, double *out_dev, void *stream);
+
+/* ------------------------------------------------------------------------
  * The GP loop on the device.  Replaces the host loop of the reference's
  * generation engine -- evolve_one (engine/gp_engine.cu:66-84) and the
  * seed-batched exhaust_batched (engine/gp_engine.cu:131-183) -- whose step
@@ -323,6 +348,13 @@ sigp_status sigp_gp_set_surrogate(sigp_gp *gp, sigp_mlp *net, const sigp_situati
 sigp_status sigp_gp_run(sigp_gp *gp, const double *sampled, int64_t n_rows, int32_t generations,
                         int32_t flags, double *fitness_history, int16_t *best_tokens,
                         int32_t *best_lengths, double *generation_ms);
+/* Scenario rows sampled on the device inside the loop: with this set,
+ * sigp_gp_run takes sampled = NULL (n_rows = 0) and generation g's row t of
+ * run r draws from Mt19937(run_seed[r] + (gen_offset + g) * gen_stride + t)
+ * -- the reference's schedule is run_seed[r] = seed_r * 10000, gen_stride = 1
+ * (engine/gp_engine.cu:69); gen_offset counts generations of earlier calls. */
+sigp_status sigp_gp_set_sampler(sigp_gp *gp, const sigp_sampler *smp, const int64_t *run_seed,
+                                int64_t gen_stride, int64_t gen_offset);
 /* The population: tokens (n_runs*population*SIGP_GP_MAX_NODES) and lengths. */
 sigp_status sigp_gp_population(const sigp_gp *gp, int16_t *tokens, int32_t *lengths);
 /* The population compiled as pack_trees would (engine/factory.cuh:20-40),

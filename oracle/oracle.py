@@ -29,7 +29,7 @@ def build(reference: bool = True) -> None:
     """Compile the oracle (and, where the reference sources exist, _ref: the
     reference's CPU simulator, and its GPU gate and tools linked against
     libsigp.so)."""
-    targets = ["liboracle.so", "liboracle_count.so"] + (["ref", "gate", "tools"] if reference else [])
+    targets = ["liboracle.so", "liboracle_count.so", "libcrmath.so"] + (["ref", "gate", "tools"] if reference else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 
@@ -234,3 +234,29 @@ def need_rows_for(pk, masks, mask_group=None) -> np.ndarray:
         if out[p, 34]:
             out[p, 30] = 1
     return out
+
+
+class CrMath:
+    """The device sampler's correctly rounded log and sincos on the host
+    (libcrmath.so): the functions injected into the host sampler to restate
+    the device's rows."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "libcrmath.so")
+        if not os.path.exists(path):
+            build(reference=False)
+        L = C.CDLL(path)
+        L.orc_cr_log.argtypes = [C.c_double]
+        L.orc_cr_log.restype = C.c_double
+        L.orc_cr_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_cr_sincos.restype = None
+        self.L = L
+        self._s = C.c_double()
+        self._c = C.c_double()
+
+    def log(self, x: float) -> float:
+        return self.L.orc_cr_log(x)
+
+    def sincos(self, x: float):
+        self.L.orc_cr_sincos(x, C.byref(self._s), C.byref(self._c))
+        return self._s.value, self._c.value

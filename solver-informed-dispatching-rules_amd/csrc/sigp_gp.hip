@@ -36,8 +36,10 @@
 #include "../../include/sigp.h"
 #include "sigp_internal.h"
 #include "sigp_rules.h"
+#include "sigp_mt.h"
 
 using namespace sigp;
+using namespace sigp_mt;
 
 #define GP_TRY(x)                                                                                   \
     do {                                                                                            \
@@ -58,7 +60,6 @@ constexpr int MAX_TOURN = 64;
 constexpr int MAX_POP = 4096;         // the breeding wavefront keeps fitness and tree words in LDS
 constexpr int TOK_FUNC = SIGP_GP_TOK_FUNC;
 constexpr int TOK_CONST = SIGP_GP_TOK_CONST;
-constexpr int MT_WORDS = 625;         // 624 state words + position
 
 struct GpParams {
     int nt, nf;
@@ -111,80 +112,6 @@ struct TreeSet {
     float *cost;
 };
 
-// ---------------------------------------------------------------------------
-// CPython's Mersenne Twister (engine/rng.hpp:54-93), driven by the whole
-// wavefront in lockstep: the 624 state words in LDS, and a window of 64
-// consecutive output words tempered in registers, lane l holding word
-// base + l, so that a draw is one read of a lane.
-
-// Regenerate the 624 words.  Word i reads words i + 1 (old) and i + 397
-// (old for i < 227) or i - 227 (already new): in chunks of 64 ascending
-// indices every such new word lies in an earlier chunk, so the chunks run
-// lane-parallel with loads and stores separated by barriers.
-__device__ __noinline__ void mt_twist(uint32_t *mt) {
-    const int lane = (int)threadIdx.x;
-    for (int base = 0; base < 623; base += 64) {
-        const int i = base + lane;
-        uint32_t nv = 0;
-        if (i < 623) {
-            const uint32_t y = (mt[i] & 0x80000000u) | (mt[i + 1] & 0x7fffffffu);
-            nv = mt[i < 227 ? i + 397 : i - 227] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-        }
-        __syncthreads();
-        if (i < 623) mt[i] = nv;
-        __syncthreads();
-    }
-    if (lane == 0) {
-        const uint32_t y = (mt[623] & 0x80000000u) | (mt[0] & 0x7fffffffu);
-        mt[623] = mt[396] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ uint32_t temper(uint32_t y) {
-    y ^= y >> 11;
-    y ^= (y << 7) & 0x9d2c5680u;
-    y ^= (y << 15) & 0xefc60000u;
-    y ^= y >> 18;
-    return y;
-}
-
-struct Rng {
-    uint32_t *mt;    // LDS state
-    int idx;         // next state word; 624 = all used, regenerate first
-    int base;        // words base .. base + 63 are tempered in `win`
-    uint32_t win;
-
-    __device__ __forceinline__ uint32_t next() {
-        if (idx >= 624 || idx - base >= 64) {
-            if (idx >= 624) {
-                mt_twist(mt);
-                idx = 0;
-            }
-            base = idx;
-            const int w = base + (int)threadIdx.x;
-            win = w < 624 ? temper(mt[w]) : 0u;
-        }
-        const int k = idx - base;
-        ++idx;
-        return (uint32_t)__builtin_amdgcn_readlane((int)win, k);
-    }
-    // the 53-bit integer m of random() = m / 2^53
-    __device__ __forceinline__ uint64_t bits53() {
-        const uint64_t a = next() >> 5, b = next() >> 6;
-        return (a << 26) | b;
-    }
-    __device__ __forceinline__ double random() { return (double)bits53() * (1.0 / 9007199254740992.0); }
-    // randbelow: rejection on bit_length(n) top bits; 0 for n == 0
-    __device__ __forceinline__ uint32_t below(uint32_t n) {
-        if (n == 0) return 0;
-        const int k = 32 - __clz(n);
-        uint32_t r = next() >> (32 - k);
-        while (r >= n) r = next() >> (32 - k);
-        return r;
-    }
-};
-
 // The ERC label "%.3f" % random() read back by compile (llround(v * 1000),
 // engine/vocab.hpp:101-104) is k = m * 1000 / 2^53 rounded to nearest, ties
 // to even (how printf rounds an exactly representable tie) -- computed
@@ -231,21 +158,6 @@ __device__ __forceinline__ int generate(const GpParams &P, Rng &g, int depth, bo
         ++cnt;
     }
     return cnt;
-}
-
-__device__ __forceinline__ void load_rng(Rng &g, uint32_t *mt, const uint32_t *state) {
-    for (int k = (int)threadIdx.x; k < 624; k += 64) mt[k] = state[k];
-    __syncthreads();
-    g.mt = mt;
-    g.idx = (int)state[624];
-    g.base = -1000;   // the window is filled at the first draw
-    g.win = 0;
-}
-
-__device__ __forceinline__ void store_rng(const Rng &g, uint32_t *state) {
-    __syncthreads();
-    for (int k = (int)threadIdx.x; k < 624; k += 64) state[k] = g.mt[k];
-    if (threadIdx.x == 0) state[624] = (uint32_t)g.idx;
 }
 
 // engine/gp.hpp:67-77: depths ramp min..max, alternating full/grow blocks.
@@ -751,41 +663,6 @@ __global__ void set_kernel(int *gen, int v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *gen = v;
 }
 
-// CPython's seeding of an integer (engine/rng.hpp:21-52 init_by_array with
-// the 32-bit words of the seed, least significant first).
-void mt_seed(uint64_t seed, uint32_t *st) {
-    std::vector<uint32_t> key;
-    if (seed == 0) key.push_back(0);
-    while (seed) {
-        key.push_back((uint32_t)(seed & 0xffffffffu));
-        seed >>= 32;
-    }
-    uint32_t *mt = st;
-    mt[0] = 19650218u;
-    for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
-    size_t i = 1, j = 0;
-    for (size_t k = std::max<size_t>(624, key.size()); k; --k) {
-        mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
-        ++i;
-        ++j;
-        if (i >= 624) {
-            mt[0] = mt[623];
-            i = 1;
-        }
-        if (j >= key.size()) j = 0;
-    }
-    for (size_t k = 623; k; --k) {
-        mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
-        ++i;
-        if (i >= 624) {
-            mt[0] = mt[623];
-            i = 1;
-        }
-    }
-    mt[0] = 0x80000000u;
-    st[624] = 624;
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -816,6 +693,10 @@ struct sigp_gp {
     sigp_mlp *net = nullptr;          // the screen (not owned)
     char *sbuf = nullptr;             // the decision situations
     sigp_device_situations sit{};
+    bool sampling = false;            // rows sampled on the device (sigp_gp_set_sampler)
+    sigp_sampler smp{};
+    long long *run_seed = nullptr;    // device: n_runs
+    long long gen_stride = 0, gen_offset = 0;
     hipStream_t stream = nullptr;
     bool ready = false;               // a population exists
 };
@@ -1095,6 +976,7 @@ extern "C" sigp_status sigp_gp_destroy(sigp_gp *g) {
     if (g->stream) (void)hipStreamDestroy(g->stream);
     if (g->dbuf) (void)hipFree(g->dbuf);
     if (g->sbuf) (void)hipFree(g->sbuf);
+    if (g->run_seed) (void)hipFree(g->run_seed);
     delete g;
     return SIGP_OK;
 }
@@ -1144,9 +1026,14 @@ extern "C" sigp_status sigp_gp_step(sigp_gp *g, const double *fitness) {
 static sigp_status launch_generation(sigp_gp *g, const double *all, double *hist, int16_t *best_tok,
                                      int32_t *best_len, bool step) {
     const long long per_gen = (long long)g->runs * g->T * g->n;
-    rows_kernel<<<(unsigned)std::min<long long>((per_gen + 255) / 256, 1024), 256, 0, g->stream>>>(all, g->gen,
-                                                                                                   per_gen, g->scen);
-    sigp_status st = sigp_batch_run(g->bt, g->scen, (int64_t)g->runs * g->T, g->pairs, g->fit, g->stream);
+    sigp_status st = SIGP_OK;
+    if (all)
+        rows_kernel<<<(unsigned)std::min<long long>((per_gen + 255) / 256, 1024), 256, 0, g->stream>>>(
+            all, g->gen, per_gen, g->scen);
+    else if ((st = sigp_internal_sample(g->inst, &g->smp, g->run_seed, 0, g->T, g->gen_stride, g->gen_offset, g->gen,
+                                        g->runs * g->T, g->scen, g->stream)) != SIGP_OK)
+        return st;
+    st = sigp_batch_run(g->bt, g->scen, (int64_t)g->runs * g->T, g->pairs, g->fit, g->stream);
     if (st != SIGP_OK) return st;
     record_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->fit, g->gen, g->P, hist);
     best_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->fit, g->ptok, g->plen, g->gen, best_tok, best_len);
@@ -1160,20 +1047,24 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
                                    int32_t flags, double *fitness_history, int16_t *best_tokens,
                                    int32_t *best_lengths, double *generation_ms) {
     const bool use_graph = (flags & SIGP_GP_GRAPH) != 0, step_last = (flags & SIGP_GP_STEP_LAST) != 0;
-    if (!g || !sampled) return gp_fail("NULL argument");
+    if (!g) return gp_fail("NULL argument");
     sigp_status st0 = check_step(g);
     if (st0 != SIGP_OK) return st0;
     if (generations < 1) return gp_fail("generations must be >= 1");
     const long long rows_per_gen = (long long)g->runs * g->T;
-    if (n_rows != rows_per_gen * generations)
+    if (!sampled) {
+        if (!g->sampling) return gp_fail("sampled is NULL and no device sampler is set");
+        if (n_rows != 0) return gp_fail("n_rows must be 0 with the device sampler");
+    } else if (n_rows != rows_per_gen * generations) {
         return gp_fail("need generations x n_runs x T = " + std::to_string(rows_per_gen * generations) + " rows");
+    }
     const size_t sbytes = (size_t)n_rows * g->n * 8, hbytes = (size_t)generations * g->P * 8,
                  nbest = (size_t)generations * g->runs;
     char *dev = nullptr;
     const size_t o_hist = (sbytes + 255) & ~(size_t)255, o_btok = o_hist + ((hbytes + 255) & ~(size_t)255),
                  o_blen = o_btok + ((nbest * MAXL * 2 + 255) & ~(size_t)255), total = o_blen + nbest * 4;
     GP_TRY(hipMalloc((void **)&dev, total));
-    double *all = (double *)dev, *hist = (double *)(dev + o_hist);
+    double *all = sampled ? (double *)dev : nullptr, *hist = (double *)(dev + o_hist);
     int16_t *btok = (int16_t *)(dev + o_btok);
     int32_t *blen = (int32_t *)(dev + o_blen);
     hipGraph_t graph = nullptr;
@@ -1187,7 +1078,8 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
     };
     for (auto &e : ev)
         if (!check(hipEventCreate(&e), "event")) break;
-    if (st == SIGP_OK && check(hipMemcpyAsync(all, sampled, sbytes, hipMemcpyHostToDevice, g->stream), "upload")) {
+    if (st == SIGP_OK && (!sampled || check(hipMemcpyAsync(all, sampled, sbytes, hipMemcpyHostToDevice, g->stream),
+                                            "upload"))) {
         set_kernel<<<1, 64, 0, g->stream>>>(g->gen, 0);
         if (use_graph && (generations > 1 || step_last)) {
             // capture one stepping generation; the counter it reads advances on the device
@@ -1226,6 +1118,23 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
         if (e) (void)hipEventDestroy(e);
     (void)hipFree(dev);
     return st;
+}
+
+extern "C" sigp_status sigp_gp_set_sampler(sigp_gp *g, const sigp_sampler *smp, const int64_t *run_seed,
+                                           int64_t gen_stride, int64_t gen_offset) {
+    if (!g || !smp || !run_seed) return gp_fail("NULL argument");
+    if (smp->mode != SIGP_SAMPLE_GAUSS && smp->mode != SIGP_SAMPLE_GAMMA) return gp_fail("bad sampler mode");
+    if (smp->mode == SIGP_SAMPLE_GAMMA && !(smp->shape >= 1.0)) return gp_fail("the device Gamma sampler needs shape >= 1");
+    for (int r = 0; r < g->runs; ++r)
+        if (run_seed[r] < 0) return gp_fail("run seeds must be >= 0");
+    GP_TRY(hipStreamSynchronize(g->stream));
+    if (!g->run_seed) GP_TRY(hipMalloc((void **)&g->run_seed, (size_t)g->runs * sizeof(long long)));
+    GP_TRY(hipMemcpy(g->run_seed, run_seed, (size_t)g->runs * sizeof(long long), hipMemcpyHostToDevice));
+    g->smp = *smp;
+    g->gen_stride = gen_stride;
+    g->gen_offset = gen_offset;
+    g->sampling = true;
+    return SIGP_OK;
 }
 
 extern "C" sigp_status sigp_gp_population(const sigp_gp *g, int16_t *tokens, int32_t *lengths) {

@@ -43,6 +43,13 @@ sigp_status sigp_internal_phenotype_device(const sigp_batch *bt, const sigp_devi
                                            int32_t *out_dev, hipStream_t stream);
 
 int sigp_internal_instance_n(const sigp_instance *inst);
+const double *sigp_internal_instance_dur(const sigp_instance *inst);   // device: expected durations
+
+// Scenario rows on `stream` (csrc/sigp_sample.hip): row r of `rows` draws from
+// Mt19937(group_seed[r / T] (seed0 if null) + (offset + *gen (0 if null)) * stride + r % T).
+sigp_status sigp_internal_sample(const sigp_instance *inst, const sigp_sampler *smp, const long long *group_seed,
+                                 long long seed0, int T, long long stride, long long offset, const int *gen,
+                                 int rows, double *out, hipStream_t stream);
 
 // Surrogate scores of `rows` rules from device-resident phenotype ranks
 // (int32 [rows][SIGP_MLP_IN]) into y (device floats), on `stream`; the same

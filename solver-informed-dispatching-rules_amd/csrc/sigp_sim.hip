@@ -1678,6 +1678,7 @@ extern "C" sigp_status sigp_device_math(int32_t which, const double *x, const do
 // interfaces for the library's other translation units
 
 int sigp_internal_instance_n(const sigp_instance *inst) { return inst->n; }
+const double *sigp_internal_instance_dur(const sigp_instance *inst) { return (const double *)(uintptr_t)inst->dev.dur; }
 
 sigp_status sigp_internal_batch_device(sigp_instance *inst, const sigp_device_rules &r, int T, int group_size,
                                        const sigp_launch_config &cfg, sigp_batch **out) {

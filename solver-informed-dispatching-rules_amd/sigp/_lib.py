@@ -9,7 +9,8 @@ import os
 from typing import Optional
 
 __all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "Situations",
-           "GpConfig", "EXPORTED", "GP_MAX_NODES", "GP_TOK_FUNC", "GP_TOK_CONST",
+           "GpConfig", "Sampler", "EXPORTED", "GP_MAX_NODES", "GP_TOK_FUNC", "GP_TOK_CONST",
+           "SAMPLE_GAUSS", "SAMPLE_GAMMA",
            "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -24,7 +25,8 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
             "sigp_mlp_predict", "sigp_gp_create", "sigp_gp_destroy", "sigp_gp_init", "sigp_gp_evaluate",
             "sigp_gp_step", "sigp_gp_run", "sigp_gp_population", "sigp_gp_packed", "sigp_gp_rng_state",
-            "sigp_gp_set_surrogate"]
+            "sigp_gp_set_surrogate", "sigp_sample_durations", "sigp_sample_durations_async",
+            "sigp_gp_set_sampler"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
            4: "SIGP_NO_DEVICE"}
@@ -65,6 +67,13 @@ class GpConfig(C.Structure):
                 ("depth_limit", C.c_int32), ("tournament_size", C.c_int32), ("elitism", C.c_int32),
                 ("crossover_rate", C.c_double), ("mutation_rate", C.c_double), ("T", C.c_int32),
                 ("launch", LaunchConfig)]
+
+
+SAMPLE_GAUSS, SAMPLE_GAMMA = 0, 1
+
+
+class Sampler(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("sigma", C.c_double), ("shape", C.c_double)]
 
 
 class Rules(C.Structure):
@@ -124,6 +133,9 @@ def lib() -> C.CDLL:
         "sigp_gp_step": ([vp, vp], C.c_int),
         "sigp_gp_run": ([vp, vp, i64, i32, i32, vp, vp, vp, vp], C.c_int),
         "sigp_gp_set_surrogate": ([vp, vp, vp], C.c_int),
+        "sigp_gp_set_sampler": ([vp, vp, vp, i64, i64], C.c_int),
+        "sigp_sample_durations": ([vp, vp, i64, i64, vp], C.c_int),
+        "sigp_sample_durations_async": ([vp, vp, i64, i64, vp, vp], C.c_int),
         "sigp_gp_population": ([vp, vp, vp], C.c_int),
         "sigp_gp_packed": ([vp, vp, vp, vp], C.c_int),
         "sigp_gp_rng_state": ([vp, i32, i32, vp], C.c_int),

@@ -15,7 +15,7 @@ from .instance import Instance
 from .rules import N_TERMS, Packed
 
 __all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info", "evaluate_multi",
-           "batch_multi", "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
+           "batch_multi", "sample_durations", "sampler", "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -230,3 +230,22 @@ def evaluate_multi(ctxs: Sequence[FactoryContext], pk: Packed, blocks: Sequence[
     check(lib().sigp_evaluate_multi(hs, K, C.byref(r), _ptr(flat), T, _ptr(m), m.shape[0],
                                     mask_group or max(P, 1), _ptr(out), _ptr(mean)))
     return (out, mean) if want_mean else out
+
+
+def sampler(mode: str = "gauss", sigma: float = 0.2, shape: float = 4.0):
+    """The sampler description of include/sigp.h (sigp_sampler)."""
+    from ._lib import SAMPLE_GAMMA, SAMPLE_GAUSS, Sampler
+    if mode not in ("gauss", "gamma"):
+        raise ValueError("mode must be 'gauss' or 'gamma'")
+    return Sampler(SAMPLE_GAUSS if mode == "gauss" else SAMPLE_GAMMA, float(sigma), float(shape))
+
+
+def sample_durations(ctx: "FactoryContext", n_rows: int, base_seed: int, mode: str = "gauss",
+                     sigma: float = 0.2, shape: float = 4.0) -> np.ndarray:
+    """Scenario rows sampled on the device (sigp_sample_durations): row r
+    draws from Mt19937(base_seed + r), as gen_durations
+    (engine/sim_host.hpp:50-58) with correctly rounded log/sin/cos."""
+    out = np.zeros((max(n_rows, 0), ctx.inst.n), dtype=np.float64)
+    smp = sampler(mode, sigma, shape)
+    check(lib().sigp_sample_durations(ctx._h, C.byref(smp), int(base_seed), int(n_rows), _ptr(out)))
+    return out

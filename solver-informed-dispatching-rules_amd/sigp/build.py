@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
          "-Wno-unused-function"]
-SOURCES = ["sigp_sim.hip", "sigp_mlp.hip", "sigp_gp.hip"]
+SOURCES = ["sigp_sim.hip", "sigp_mlp.hip", "sigp_gp.hip", "sigp_sample.hip"]
 
 
 def build(verbose: bool = False) -> str:

@@ -145,6 +145,19 @@ class DeviceGP:
         check(lib().sigp_gp_set_surrogate(self._h, net._h, C.byref(s)))
         self._net = net
 
+    def set_sampler(self, run_seeds: Sequence[int], mode: str = "gamma", sigma: float = 0.2, shape: float = 4.0,
+                    gen_stride: int = 1, gen_offset: int = 0) -> None:
+        """Sample each generation's rows on the device inside the loop:
+        row t of run r in generation g draws from
+        Mt19937(run_seeds[r] + (gen_offset + g) * gen_stride + t); `run`
+        then takes sampled_all = None."""
+        from .api import sampler
+        smp = sampler(mode, sigma, shape)
+        seeds = np.ascontiguousarray([int(v) for v in run_seeds], dtype=np.int64)
+        if seeds.size != self.runs:
+            raise ValueError("need one seed per run")
+        check(lib().sigp_gp_set_sampler(self._h, C.byref(smp), _ptr(seeds), int(gen_stride), int(gen_offset)))
+
     def init(self) -> None:
         """ramped_half_and_half of every run (engine/gp.hpp:67-77)."""
         check(lib().sigp_gp_init(self._h))
@@ -178,8 +191,11 @@ class DeviceGP:
         """The whole loop on the device: generation g evaluates on rows
         g*runs*T .. and steps, except the last unless `step_last` (a loop
         run in several calls).  Generations are numbered from this call."""
-        rows = generations * self.runs * self.T
-        a = self._rows(sampled_all, rows)
+        if sampled_all is None:
+            rows, a = 0, None
+        else:
+            rows = generations * self.runs * self.T
+            a = self._rows(sampled_all, rows)
         hist = np.zeros(generations * self.runs * self.pop, dtype=np.float64)
         btok = np.zeros(generations * self.runs * GP_MAX_NODES, dtype=np.int16)
         blen = np.zeros(generations * self.runs, dtype=np.int32)

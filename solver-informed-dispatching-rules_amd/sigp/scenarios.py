@@ -33,16 +33,22 @@ _TWOPI = 2.0 * 3.14159265358979323846
 
 
 class GaussStream:
-    """Mt19937 + cached Box-Muller gauss (engine/rng.hpp:135-147)."""
+    """Mt19937 + cached Box-Muller gauss (engine/rng.hpp:135-147).
 
-    __slots__ = ("rng", "has", "spare", "_s", "_c")
+    `math`, when given, supplies the log and sincos used instead of libm's
+    (an object with .log(x) and .sincos(x) -> (sin, cos)); the device sampler
+    is this stream with correctly rounded functions."""
 
-    def __init__(self, seed: int):
+    __slots__ = ("rng", "has", "spare", "_s", "_c", "log", "_math")
+
+    def __init__(self, seed: int, math_fns=None):
         self.rng = random.Random(seed)
         self.has = False
         self.spare = 0.0
         self._s = ctypes.c_double()
         self._c = ctypes.c_double()
+        self._math = math_fns
+        self.log = math.log if math_fns is None else math_fns.log
 
     def random(self) -> float:
         return self.rng.random()
@@ -53,10 +59,14 @@ class GaussStream:
             self.has = False
         else:
             x2pi = self.rng.random() * _TWOPI
-            g2rad = math.sqrt(-2.0 * math.log(1.0 - self.rng.random()))
-            _libm.sincos(x2pi, ctypes.byref(self._s), ctypes.byref(self._c))
-            z = self._c.value * g2rad
-            self.spare = self._s.value * g2rad
+            g2rad = math.sqrt(-2.0 * self.log(1.0 - self.rng.random()))
+            if self._math is None:
+                _libm.sincos(x2pi, ctypes.byref(self._s), ctypes.byref(self._c))
+                sn, cs = self._s.value, self._c.value
+            else:
+                sn, cs = self._math.sincos(x2pi)
+            z = cs * g2rad
+            self.spare = sn * g2rad
             self.has = True
         return mu + z * sigma
 
@@ -79,18 +89,19 @@ def gamma_draw(rng: GaussStream, shape: float, scale: float) -> float:
         u = rng.random()
         if u < 1.0 - 0.0331 * (x * x) * (x * x):
             return scale * d * v
-        if u > 0.0 and math.log(u) < 0.5 * x * x + d * (1.0 - v + math.log(v)):
+        if u > 0.0 and rng.log(u) < 0.5 * x * x + d * (1.0 - v + rng.log(v)):
             return scale * d * v
 
 
 def gen_durations(expected: np.ndarray, n_samples: int, base_seed: int, sigma: float = 0.2,
-                  mode: str = "gauss", shape: float = 4.0) -> np.ndarray:
-    """n_samples x n row-major array (scenario-major), one generator per scenario."""
+                  mode: str = "gauss", shape: float = 4.0, math_fns=None) -> np.ndarray:
+    """n_samples x n row-major array (scenario-major), one generator per
+    scenario; `math_fns` replaces libm's log and sincos (see GaussStream)."""
     mu = [float(v) for v in np.asarray(expected, dtype=np.float64)]
     n = len(mu)
     out = np.zeros((n_samples, n), dtype=np.float64)
     for s in range(n_samples):
-        rng = GaussStream(base_seed + s)
+        rng = GaussStream(base_seed + s, math_fns)
         row = out[s]
         for i, e in enumerate(mu):
             if e <= 0.0:
