@@ -189,6 +189,58 @@ def bench_gp_loop(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sampler(args):
+    """Scenario rows on the device: the large configuration's 2,000 Gamma(2)
+    rows of n = 288 (clustered + uniform, m = 4), each row its own
+    Mt19937(1,000,000 + r) stream.  A step is one full set of rows; the
+    reference samples them on the host (engine/sim_host.hpp:50-58), timed
+    beside it on the same rows with the host restatement."""
+    import torch
+    from sigp.api import FactoryContext, device_info, sampler
+    from sigp._lib import check, lib
+    from sigp.instance import synthetic_mixed
+    from sigp.scenarios import gen_durations
+    import ctypes as C
+    rows = args.scenarios or 2000
+    inst = synthetic_mixed(n=288, m=4, seed=1)
+    ctx = FactoryContext(inst)
+    name, n_cu, _ = device_info()
+    log("device", name, n_cu, "CUs")
+    smp = sampler("gamma", 0.2, 2.0)
+    out = torch.zeros(rows * inst.n, dtype=torch.float64, device="cuda:0")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        check(lib().sigp_sample_durations_async(ctx._h, C.byref(smp), 1_000_000, rows,
+                                                C.c_void_p(out.data_ptr()), C.c_void_p(stream.cuda_stream)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    for a, b in evs:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize()
+    ms = float(np.mean([a.elapsed_time(b) for a, b in evs]))
+    sample = min(rows, 100)
+    t0 = time.perf_counter()
+    gen_durations(inst.durations, sample, 1_000_000, mode="gamma", shape=2.0)
+    host_s = time.perf_counter() - t0
+    out_d = {"metric": "scenario rows/sec (device sampler)", "value": round(rows / (ms / 1000.0), 1),
+             "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+             "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+             "dtype": "f64", "data": "synthetic",
+             "config": {"workload": "device sampler: %d Gamma(2, mean 10) rows of n=%d (clustered + uniform "
+                                    "instance, m=4), one Mersenne Twister stream per row" % (rows, inst.n),
+                        "rows": rows, "n": inst.n, "parallelism": "1 GPU; one wavefront per row"},
+             "roofline": None,
+             "cpu_baseline": {"value": round(sample / host_s, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+                              "sample": "the first %d rows with the host sampler (Python, libm)" % sample}}
+    print(json.dumps(out_d), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,7 +249,7 @@ def main():
     ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
     ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
     ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large",
-                                           "gp-loop"],
+                                           "gp-loop", "sampler"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
@@ -205,7 +257,8 @@ def main():
                          "rules x 1 scenario); batched: 40 uneven instances (n 48-288) x 256 "
                          "rules x 500 scenarios in one multi-instance batch; large: n=288 m=4, "
                          "4,096 rules x 2,000 scenarios; gp-loop: the whole GP loop on the "
-                         "device (1,024 rules x 50 scenarios x 50 generations, breeding included)")
+                         "device (1,024 rules x 50 scenarios x 50 generations, breeding included); "
+                         "sampler: 2,000 Gamma(2) scenario rows of n=288 sampled on the device")
     ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
     ap.add_argument("--runs", type=int, default=1, help="gp-loop: independent runs side by side")
     ap.add_argument("--no-graph", action="store_true", help="gp-loop: stream launches, no HIP graph")
@@ -225,6 +278,8 @@ def main():
 
     if args.workload == "gp-loop":
         return bench_gp_loop(args)
+    if args.workload == "sampler":
+        return bench_sampler(args)
     import torch
     from sigp.api import FactoryContext, batch_multi, device_info
     from sigp.workloads import (batched_uneven, gp_generation, gp_reference_generation, large,
