@@ -298,7 +298,8 @@ typedef struct {
     const int32_t *terminal_slots;   /* slot of each terminal (0..60), -1 = "ERC" */
     int32_t n_functions;             /* TreeConfig::functions, <= 8 */
     const int32_t *function_ops;     /* opcode of each function (0..6, engine/vocab.hpp:79-88) */
-    int32_t min_depth, max_depth;    /* TreeConfig: initialisation and mutation depths; mutants deeper than max_depth are rejected */
+    int32_t min_depth, max_depth;    /* TreeConfig: initialisation and mutation depths (mutants deeper
+                                        than max_depth are rejected) */
     int32_t depth_limit;             /* EvolutionConfig::max_depth: crossover children deeper are rejected (<= 8) */
     int32_t tournament_size;         /* <= 64 */
     int32_t elitism;

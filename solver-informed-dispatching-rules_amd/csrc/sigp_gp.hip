@@ -1087,7 +1087,8 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
                 const sigp_status s2 = launch_generation(g, all, hist, btok, blen, true);
                 const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
                 if (s2 != SIGP_OK && st == SIGP_OK) st = s2;
-                if (check(ec, "end capture")) check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
+                if (check(ec, "end capture"))
+                    check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "instantiate");
             }
         }
         if (st == SIGP_OK) check(hipEventRecord(ev[0], g->stream), "event");
@@ -1107,9 +1108,12 @@ extern "C" sigp_status sigp_gp_run(sigp_gp *g, const double *sampled, int64_t n_
                 generation_ms[gi] = ms;
             }
         }
-        if (st == SIGP_OK && fitness_history) check(hipMemcpy(fitness_history, hist, hbytes, hipMemcpyDeviceToHost), "download");
-        if (st == SIGP_OK && best_tokens) check(hipMemcpy(best_tokens, btok, nbest * MAXL * 2, hipMemcpyDeviceToHost), "download");
-        if (st == SIGP_OK && best_lengths) check(hipMemcpy(best_lengths, blen, nbest * 4, hipMemcpyDeviceToHost), "download");
+        auto down = [&](void *dst, const void *src, size_t bytes) {
+            if (st == SIGP_OK && dst) check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
+        };
+        down(fitness_history, hist, hbytes);
+        down(best_tokens, btok, nbest * MAXL * 2);
+        down(best_lengths, blen, nbest * 4);
     }
     (void)hipStreamSynchronize(g->stream);
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -1124,7 +1128,8 @@ extern "C" sigp_status sigp_gp_set_sampler(sigp_gp *g, const sigp_sampler *smp, 
                                            int64_t gen_stride, int64_t gen_offset) {
     if (!g || !smp || !run_seed) return gp_fail("NULL argument");
     if (smp->mode != SIGP_SAMPLE_GAUSS && smp->mode != SIGP_SAMPLE_GAMMA) return gp_fail("bad sampler mode");
-    if (smp->mode == SIGP_SAMPLE_GAMMA && !(smp->shape >= 1.0)) return gp_fail("the device Gamma sampler needs shape >= 1");
+    if (smp->mode == SIGP_SAMPLE_GAMMA && !(smp->shape >= 1.0))
+        return gp_fail("the device Gamma sampler needs shape >= 1");
     for (int r = 0; r < g->runs; ++r)
         if (run_seed[r] < 0) return gp_fail("run seeds must be >= 0");
     GP_TRY(hipStreamSynchronize(g->stream));
