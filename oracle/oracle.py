@@ -29,7 +29,7 @@ def build(reference: bool = True) -> None:
     """Compile the oracle (and, where the reference sources exist, _ref: the
     reference's CPU simulator, and its GPU gate and tools linked against
     libsigp.so)."""
-    targets = ["liboracle.so", "liboracle_count.so", "libcrmath.so"] + (["ref", "gate", "tools"] if reference else [])
+    targets = ["liboracle.so", "liboracle_count.so", "libcrmath.so"] + (["ref", "gate", "tools", "gpcheck"] if reference else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 

@@ -6,6 +6,9 @@ reference's stored results on MI355X:
   seed's final rule and test gain, every generation's best training fitness,
   size and extra terminals) equals results/ga40/results/trace_<inst>_Sga.json
   field for field;
+* include/sigp_gp.hpp: evolve_one_device (the GP loop on the device) returns
+  the population of the reference's own generation loop (its engine/gp.hpp
+  operators on the host) tree for tree;
 * s4_selection/ga.cu: the proxy fitness of the GA's initial pool (19 terminal
   subsets x 16 seeds x 51 generations) equals the hexfloats of the stored
   evaluation cache results/ga40/logs/c204.evcache, except the subset
@@ -92,3 +95,18 @@ def test_ga_initial_pool_reproduces_stored_cache(tmp_path):
         assert g in got and g in want
         if g != atrisk:
             assert got[g] == want[g], hex(g)
+
+
+@pytest.mark.parametrize("args", [["c101"], ["rc101", "--terms", "REACH2,FRAGCNT", "--seed", "3", "--samples", "5",
+                                              "--gens", "20"]],
+                         ids=["c101-ns11-51gens", "rc101-ext-20gens"])
+def test_evolve_one_device_equals_the_reference_loop(args, tmp_path):
+    """include/sigp_gp.hpp's evolve_one_device against the reference's own
+    generation loop (engine/gp.hpp operators on the host, FactoryContext
+    evaluation): identical final populations (oracle/gp_swap_check.cpp)."""
+    path = os.path.join(ROOT, "oracle", "_ref", "gp_swap_check")
+    if not os.path.exists(path):
+        pytest.skip("gp_swap_check not built (needs the reference sources at build time)")
+    res = subprocess.run([path] + args, cwd=tmp_path, env=_env(), capture_output=True, text=True, timeout=300)
+    print(res.stdout, flush=True)
+    assert res.returncode == 0, res.stdout + res.stderr[-2000:]
