@@ -307,7 +307,26 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
 // renumbered when the batch was built.
 // RARE = false compiles out the slots no scan family produces (44, 49, 50,
 // 53-57, 59, 60) for batches whose rules read none of them.
-template <int NW, bool RMOD, bool RARE>
+// Index of the r-th (0-based) set bit of a, r < popcount(a).
+__device__ __forceinline__ int select_bit(uint32_t a, int r) {
+    int pos = 0;
+    int c = __builtin_popcount(a & 0xffffu);
+    if (r >= c) { r -= c; a >>= 16; pos += 16; }
+    c = __builtin_popcount(a & 0xffu);
+    if (r >= c) { r -= c; a >>= 8; pos += 8; }
+    c = __builtin_popcount(a & 0xfu);
+    if (r >= c) { r -= c; a >>= 4; pos += 4; }
+    c = __builtin_popcount(a & 0x3u);
+    if (r >= c) { r -= c; a >>= 2; pos += 2; }
+    if (r >= (int)(a & 1u)) pos += 1;
+    return pos;
+}
+
+// COMPACT: the route state (cur, ct, vis) is the same in every lane of the
+// workgroup (candidate lanes), so the feasible candidates are too, and the
+// k-th of them (in index order) goes to the lanes' k-th slot: k = i0,
+// i0 + istep, ... -- no lane is handed an infeasible candidate.
+template <int NW, bool RMOD, bool RARE, bool COMPACT = false>
 __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
                       uint64_t rd, uint64_t fl, int cur, double ct, int rem,
                       const uint32_t (&vis)[NW], int i0, int istep, double *tv, double *st,
@@ -372,12 +391,28 @@ __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, cons
 
     int best = -1;
     double best_pri = -1e300;
-    for (int i = i0; i < I.n; i += istep) {
-        const uint32_t b = 1u << (i & 31);
-        const int w = i >> 5;
-        const bool feas = !(wsel<NW>(vis, w) & b) && (wsel<NW>(fnow, w) & b);
-        if (__builtin_amdgcn_ballot_w64(feas) == 0) continue;
-        if (!feas) continue;
+    const int n_feas = (int)nfeas;
+    for (int k = i0; k < (COMPACT ? n_feas : I.n); k += istep) {
+        int i = k;
+        if (COMPACT) {
+            int r = k;
+            i = -1;
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const uint32_t a = fnow[q] & ~vis[q];
+                const int c = __builtin_popcount(a);
+                if (i < 0) {
+                    if (r < c) i = q * 32 + select_bit(a, r);
+                    else r -= c;
+                }
+            }
+        } else {
+            const uint32_t b = 1u << (i & 31);
+            const int w = i >> 5;
+            const bool feas = !(wsel<NW>(vis, w) & b) && (wsel<NW>(fnow, w) & b);
+            if (__builtin_amdgcn_ballot_w64(feas) == 0) continue;
+            if (!feas) continue;
+        }
         const double ta = I.distT[(size_t)(i + 1) * I.ld + cur];
         const double oi = I.open[i], di = I.dur[i];
         const double to = oi - ct;
@@ -544,8 +579,9 @@ __device__ __forceinline__ int block_argmax(double pri, int idx, int W, int wv, 
 }
 
 // Candidate-lane mapping: one workgroup = one (rule, scenario) simulation
-// (engine/sim_core.hpp:207-238); thread t evaluates candidates t, t + 64W,
-// ... of every decision and the workgroup takes the first maximum.  Route
+// (engine/sim_core.hpp:207-238); thread t evaluates the feasible candidates
+// t, t + 64W, ... (in index order) of every decision and the workgroup takes
+// the first maximum.  Route
 // state is identical in every thread.  Suited to few scenarios per rule (the
 // reference's GP loop evaluates one per generation, engine/gp.hpp:32), where
 // scenario lanes would sit idle.  Dynamic LDS as in sim_kernel.
@@ -584,8 +620,8 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
     while (running) {
         double bp = -1e300;
         const int rem = I.m - veh - 1;
-        int best = decide<NW, RMOD, RARE>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis,
-                                          (int)threadIdx.x, (int)blockDim.x, tv, st, bp);
+        int best = decide<NW, RMOD, RARE, true>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis,
+                                                (int)threadIdx.x, (int)blockDim.x, tv, st, bp);
         best = block_argmax(bp, best, W, wv, lane, rpri, ridx);
         if (best < 0) {
             ++veh;
