@@ -338,7 +338,35 @@ __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, cons
 #pragma unroll
     for (int w = 0; w < NW; ++w) { fnow[w] = 0; cnow[w] = 0; frob[w] = 0; }
     const bool want_c = fl & F_CNOW, want_r = fl & F_FNOWROB;
-    {
+    if (COMPACT) {
+        // one customer per lane, 64 bits per ballot (the state is the same in every lane)
+        const int lane = (int)(threadIdx.x & 63);
+        for (int b = 0; b < I.n; b += 64) {
+            const int j = b + lane;
+            bool f = false, c = false, r = false;
+            if (j < I.n) {
+                const size_t at = (size_t)j * F.ldq + cur;
+                f = ct <= F.t0c[at];
+                if (want_c) c = ct <= F.t1c[at];
+                if (want_r) r = ct <= F.t2c[at];
+            }
+            const uint64_t mf = __builtin_amdgcn_ballot_w64(f), mc = __builtin_amdgcn_ballot_w64(c),
+                           mr = __builtin_amdgcn_ballot_w64(r);
+            const int w0 = b >> 5;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) {
+                if (k == w0) {
+                    fnow[k] = (uint32_t)mf;
+                    cnow[k] = (uint32_t)mc;
+                    frob[k] = (uint32_t)mr;
+                } else if (k == w0 + 1) {
+                    fnow[k] = (uint32_t)(mf >> 32);
+                    cnow[k] = (uint32_t)(mc >> 32);
+                    frob[k] = (uint32_t)(mr >> 32);
+                }
+            }
+        }
+    } else {
         uint32_t fw = 0, cw = 0, rw = 0;
         for (int j = 0; j < I.n; ++j) {
             const int bit = j & 31;
