@@ -35,6 +35,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 (vector and matrix) peak, FMA 
 # FETCH_SIZE and WRITE_SIZE counters of profiles/sim_kernel_plain_small_hbm_traffic.md,
 # each scaled by what it reads for a plain copy of known size (x1.9997, x1.0).
 TRAFFIC_BYTES = {(1024, 500): 20265718}
+MAPPINGS = {"scenarios": 1, "candidates": 2, "candidates2": 3}
 
 
 def log(*a):
@@ -142,7 +143,7 @@ def bench_gp_loop(args):
     seeds = list(range(runs))
     launch = {}
     if args.waves:
-        launch = dict(waves_per_item=args.waves, mapping=1 if args.mapping == "scenarios" else 2)
+        launch = dict(waves_per_item=args.waves, mapping=MAPPINGS[args.mapping])
     # warm-up: code objects, graph instantiation
     gp = DeviceGP(ctx, tc, ec, seeds=seeds, T=T, term_mask=mask, **launch)
     gp.init()
@@ -266,8 +267,9 @@ def main():
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--waves", type=int, default=0,
                     help="wavefronts per workgroup: 1, 2 or 4; 0 = autotune")
-    ap.add_argument("--mapping", choices=["scenarios", "candidates"], default="scenarios",
-                    help="lane mapping used with --waves")
+    ap.add_argument("--mapping", choices=sorted(MAPPINGS), default="scenarios",
+                    help="lane mapping used with --waves (candidates2: 2 simulations per wavefront, "
+                         "--waves 1)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
@@ -323,13 +325,13 @@ def main():
         batch.run(sampled.data_ptr(), n_rows, pairs.data_ptr(), means.data_ptr(), stream.cuda_stream)
 
     if args.waves:
-        mapping = 1 if args.mapping == "scenarios" else 2
+        mapping = MAPPINGS[args.mapping]
         batch.configure(args.waves, mapping)
         waves, how = args.waves, "fixed"
     else:
         log("autotuning the launch mapping ...")
         (waves, mapping), how = batch.autotune(sampled.data_ptr(), n_rows, stream.cuda_stream), "autotuned"
-    lanes = "64-scenario block" if mapping == 1 else "(rule, scenario) simulation"
+    lanes = {1: "64-scenario block", 2: "(rule, scenario) simulation", 3: "wavefront of 2 simulations"}[mapping]
     log("%s mapping: %d wavefronts per %s" % (how, waves, lanes))
 
     for _ in range(args.warmup):
@@ -399,8 +401,8 @@ def main():
                    "n": wl.inst.n if hasattr(wl, "inst") else None,
                    "m": wl.inst.m if hasattr(wl, "inst") else None,
                    "parallelism": "1 GPU; workgroup = %s, %d wavefronts (%s)"
-                                  % ("rule x 64 scenarios" if mapping == 1 else
-                                     "one (rule, scenario) simulation", waves, how)},
+                                  % ({1: "rule x 64 scenarios", 2: "one (rule, scenario) simulation",
+                                      3: "2 (rule, scenario) simulations"}[mapping], waves, how)},
         "roofline": roof, "cpu_baseline": cpu,
     }
     if cpu:

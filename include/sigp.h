@@ -146,6 +146,12 @@ sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst, con
  *     take the first maximum together.  For few scenarios per rule, such as
  *     the reference's GP loop, which evaluates one per generation
  *     (engine/gp.hpp:32).
+ *   SIGP_MAP_CANDIDATES_2: candidate lanes in two segments -- one 64-lane
+ *     wavefront simulates 2 scenarios of a rule, 32 lanes splitting each
+ *     one's candidates; waves_per_item is ignored.  For n <= 288 customers
+ *     (larger instances run as SIGP_MAP_CANDIDATES); pays where most
+ *     decisions have few feasible candidates and rules are long (evolved GP
+ *     populations at n = 100).
  *   SIGP_MAP_AUTO: candidate lanes when T < 64 or when scenario lanes would
  *     make fewer than 512 workgroups, scenario lanes otherwise.
  * Each wavefront keeps the candidates' terminal values and the postfix stack
@@ -155,6 +161,7 @@ sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst, con
 #define SIGP_MAP_AUTO 0
 #define SIGP_MAP_SCENARIOS 1
 #define SIGP_MAP_CANDIDATES 2
+#define SIGP_MAP_CANDIDATES_2 3
 typedef struct {
     int32_t waves_per_item;   /* 1, 2 or 4; 0 = default of the mapping */
     int32_t mapping;          /* SIGP_MAP_* */

@@ -325,8 +325,10 @@ __device__ __forceinline__ int select_bit(uint32_t a, int r) {
 // COMPACT: the route state (cur, ct, vis) is the same in every lane of the
 // workgroup (candidate lanes), so the feasible candidates are too, and the
 // k-th of them (in index order) goes to the lanes' k-th slot: k = i0,
-// i0 + istep, ... -- no lane is handed an infeasible candidate.
-template <int NW, bool RMOD, bool RARE, bool COMPACT = false>
+// i0 + istep, ... -- no lane is handed an infeasible candidate.  SEGW < 64:
+// the state is the same within each aligned segment of SEGW lanes (one
+// simulation per segment), and the masks are built per segment.
+template <int NW, bool RMOD, bool RARE, bool COMPACT = false, int SEGW = 64>
 __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, const cip ops, const cip args, int L,
                       uint64_t rd, uint64_t fl, int cur, double ct, int rem,
                       const uint32_t (&vis)[NW], int i0, int istep, double *tv, double *st,
@@ -338,7 +340,34 @@ __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, cons
 #pragma unroll
     for (int w = 0; w < NW; ++w) { fnow[w] = 0; cnow[w] = 0; frob[w] = 0; }
     const bool want_c = fl & F_CNOW, want_r = fl & F_FNOWROB;
-    if (COMPACT) {
+    if (COMPACT && SEGW < 64) {
+        // segments of SEGW lanes, each with its own route state: one customer
+        // per lane of the segment, the segment's SEGW bits of each ballot
+        const int lane = (int)(threadIdx.x & 63);
+        const int sl = lane % SEGW, base = lane - sl;
+        const uint64_t keep = (1ull << SEGW) - 1;
+        for (int b = 0; b < I.n; b += SEGW) {
+            const int j = b + sl;
+            bool f = false, c = false, r = false;
+            if (j < I.n) {
+                const size_t at = (size_t)j * F.ldq + cur;
+                f = ct <= F.t0c[at];
+                if (want_c) c = ct <= F.t1c[at];
+                if (want_r) r = ct <= F.t2c[at];
+            }
+            const uint32_t sf = (uint32_t)((__builtin_amdgcn_ballot_w64(f) >> base) & keep),
+                           sc = (uint32_t)((__builtin_amdgcn_ballot_w64(c) >> base) & keep),
+                           sr = (uint32_t)((__builtin_amdgcn_ballot_w64(r) >> base) & keep);
+            const int w0 = b >> 5, sh = b & 31;
+#pragma unroll
+            for (int k = 0; k < NW; ++k)
+                if (k == w0) {
+                    fnow[k] |= sf << sh;
+                    cnow[k] |= sc << sh;
+                    frob[k] |= sr << sh;
+                }
+        }
+    } else if (COMPACT) {
         // one customer per lane, 64 bits per ballot (the state is the same in every lane)
         const int lane = (int)(threadIdx.x & 63);
         for (int b = 0; b < I.n; b += 64) {
@@ -668,6 +697,88 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
         }
     }
     if (threadIdx.x == 0) out[slots[k].out_off + (size_t)p * S.T + s] = total;
+}
+
+// First maximum within each segment of SEGW lanes (the segments are
+// aligned, so the butterfly never crosses one): every lane gets its
+// segment's winner (or -1).
+template <int SEGW>
+__device__ __forceinline__ int seg_argmax(double pri, int idx) {
+#pragma unroll
+    for (int off = SEGW / 2; off >= 1; off >>= 1) {
+        const double op = __shfl_xor(pri, off);
+        const int oi = __shfl_xor(idx, off);
+        if (beats(op, oi, pri, idx)) { pri = op; idx = oi; }
+    }
+    return idx;
+}
+
+// Segmented candidate lanes: one wavefront simulates 64 / SEGW scenarios of
+// one rule, segment g (lanes g*SEGW ..) the scenario blk * (64/SEGW) + g;
+// within a segment, as in sim_cand_kernel, lane t evaluates the feasible
+// candidates t, t + SEGW, ... and the segment takes the first maximum.
+// Decisions with few feasible candidates -- most of them -- then fill the
+// wavefront with several simulations instead of one.  Workgroup = one
+// wavefront; dynamic LDS as in sim_kernel.
+template <int NW, bool RMOD, bool RARE, int SEGW>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64), amdgpu_waves_per_eu(2)))
+void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
+    extern __shared__ double lds_rows[];
+    constexpr int SEGS = 64 / SEGW;
+    const int nsb = (S.T + SEGS - 1) / SEGS;
+    const int rank = (int)blockIdx.x / nsb;
+    const int blk = (int)blockIdx.x - rank * nsb;
+    const int pair = R.order ? R.order[rank] : rank;
+    const int k = pair / R.P;
+    const int p = pair - k * R.P;
+    const DevInst I = slots[k].dev;
+    const FusedTabs F = slots[k].tabs;
+    const int lane = threadIdx.x & 63;
+    const int sl = lane % SEGW;
+    const int s = blk * SEGS + lane / SEGW;
+    const bool active = s < S.T;
+    const int off = R.off[p], L = R.len[p];
+    const cip ops = R.ops + off;
+    const cip args = R.args + 2 * (size_t)off;
+    const uint64_t rd = R.rd[p];
+    const uint64_t fl = R.flags[p];
+    const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
+    const double *__restrict__ samp = S.sampled + slots[k].samp_off + row * (size_t)I.n;
+    double *tv = lds_rows + lane;
+    double *st = tv + (size_t)R.KL * 64;
+
+    uint32_t vis[NW];
+#pragma unroll
+    for (int w = 0; w < NW; ++w) vis[w] = 0;
+    int cur = 0, veh = 0;
+    double ct = 0.0, total = 0.0;
+    bool running = active && I.m > 0;
+    while (__builtin_amdgcn_ballot_w64(running) != 0) {
+        double bp = -1e300;
+        const int rem = I.m - veh - 1;
+        // a finished segment walks no candidates (its first slot lies past any count)
+        int best = decide<NW, RMOD, RARE, true, SEGW>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis,
+                                                      running ? sl : (1 << 20), SEGW, tv, st, bp);
+        best = seg_argmax<SEGW>(bp, best);
+        if (running) {
+            if (best < 0) {
+                ++veh;
+                cur = 0;
+                ct = 0.0;
+                if (veh >= I.m) running = false;
+            } else {
+                const int pb = best + 1;
+                const double ta = I.dist[(size_t)cur * I.ld + pb];
+                const double a = ct + ta;
+                const double stt = a > I.open[best] ? a : I.open[best];
+                ct = stt + samp[best];
+                cur = pb;
+                total += I.score[best];
+                wset<NW>(vis, best);
+            }
+        }
+    }
+    if (active && sl == 0) out[slots[k].out_off + (size_t)p * S.T + s] = total;
 }
 
 // Decision situations of the phenotype: per situation the state of a route
@@ -1318,9 +1429,8 @@ static sigp_status check_config(const sigp_launch_config *cfg) {
     const int w = cfg->waves_per_item;
     if (w != 0 && w != 1 && w != 2 && w != 4)
         return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
-    if (cfg->mapping != SIGP_MAP_AUTO && cfg->mapping != SIGP_MAP_SCENARIOS &&
-        cfg->mapping != SIGP_MAP_CANDIDATES)
-        return fail(SIGP_INVALID_ARG, "mapping must be SIGP_MAP_AUTO, _SCENARIOS or _CANDIDATES");
+    if (cfg->mapping < SIGP_MAP_AUTO || cfg->mapping > SIGP_MAP_CANDIDATES_2)
+        return fail(SIGP_INVALID_ARG, "mapping must be SIGP_MAP_AUTO, _SCENARIOS, _CANDIDATES or _CANDIDATES_2");
     return SIGP_OK;
 }
 
@@ -1383,6 +1493,7 @@ static int resolved_mapping(const sigp_batch *bt, const LaunchUnit &u) {
 // 1,024 / 4,096 rules x 50 scenarios, 1 wavefront beat 2 and 4 by 15-60 %);
 // then reduced until LDS fits.
 static int resolved_waves(const sigp_batch *bt, const LaunchUnit &u, int mapping) {
+    if (mapping == SIGP_MAP_CANDIDATES_2) return 1;
     int w = u.waves ? u.waves : bt->waves_per_item;
     if (w == 0) w = mapping == SIGP_MAP_CANDIDATES ? 1 : 4;
     return fit_waves(bt, w);
@@ -1394,7 +1505,13 @@ static void launch_variant(const sigp_batch *bt, const LaunchUnit &u, int mappin
     RuleSet R = bt->rs;
     R.order = u.order;
     const unsigned lds = (unsigned)(w * lds_per_wave(bt));
-    if (mapping == SIGP_MAP_CANDIDATES)
+    if constexpr (NW <= 9) {
+        if (mapping == SIGP_MAP_CANDIDATES_2) {
+            sim_seg_kernel<NW, RMOD, RARE, 32><<<items, 64, lds, stream>>>(bt->slots, R, S, out);
+            return;
+        }
+    }
+    if (mapping != SIGP_MAP_SCENARIOS)
         sim_cand_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
     else
         sim_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
@@ -1419,9 +1536,14 @@ static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const 
     S.T = bt->T;
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
-    const int mapping = resolved_mapping(bt, u);
+    int mapping = resolved_mapping(bt, u);
+    if (u.nwb > 9 && mapping == SIGP_MAP_CANDIDATES_2)
+        mapping = SIGP_MAP_CANDIDATES;   // segments are built for n <= 288 only
     const int w = resolved_waves(bt, u, mapping);
-    const long long items = u.n_pairs * (mapping == SIGP_MAP_CANDIDATES ? (long long)S.T : (long long)S.nblk);
+    const long long items =
+        u.n_pairs * (mapping == SIGP_MAP_CANDIDATES    ? (long long)S.T
+                     : mapping == SIGP_MAP_SCENARIOS   ? (long long)S.nblk
+                                                       : (long long)((S.T + 1) / 2));
     if (items > 0xffffffffLL) return fail(SIGP_INVALID_ARG, "launch exceeds 2^32 workgroups");
     if (u.nwb == 4) launch_class<4>(bt, u, mapping, w, (unsigned)items, S, out, stream);
     else if (u.nwb == 9) launch_class<9>(bt, u, mapping, w, (unsigned)items, S, out, stream);
@@ -1487,13 +1609,15 @@ static sigp_status tune_unit(sigp_batch *bt, LaunchUnit &u, const double *sample
     probe.order = u.tune_order;
     probe.n_pairs = u.tune_pairs;
     struct Cand { int mapping, w; };
-    const Cand cands[6] = {{SIGP_MAP_SCENARIOS, 1}, {SIGP_MAP_SCENARIOS, 2}, {SIGP_MAP_SCENARIOS, 4},
-                           {SIGP_MAP_CANDIDATES, 1}, {SIGP_MAP_CANDIDATES, 2}, {SIGP_MAP_CANDIDATES, 4}};
+    const Cand cands[7] = {{SIGP_MAP_SCENARIOS, 1},  {SIGP_MAP_SCENARIOS, 2},  {SIGP_MAP_SCENARIOS, 4},
+                           {SIGP_MAP_CANDIDATES, 1}, {SIGP_MAP_CANDIDATES, 2}, {SIGP_MAP_CANDIDATES, 4},
+                           {SIGP_MAP_CANDIDATES_2, 1}};
     const int groups = (u.max_n + 63) / 64;
     int best_w = 0, best_m = SIGP_MAP_AUTO;
     float best_ms = 0.0f;
     for (const Cand &c : cands) {
         if (fit_waves(bt, c.w) != c.w) continue;                           // LDS does not fit
+        if (c.mapping == SIGP_MAP_CANDIDATES_2 && u.nwb > 9) continue;      // built for n <= 288
         if (c.mapping == SIGP_MAP_CANDIDATES && c.w > 1 && 32 * c.w >= groups * 64) continue;
         probe.mapping = c.mapping;
         probe.waves = c.w;

@@ -11,7 +11,8 @@ from typing import Optional
 __all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "LaunchConfig", "Situations",
            "GpConfig", "Sampler", "EXPORTED", "GP_MAX_NODES", "GP_TOK_FUNC", "GP_TOK_CONST",
            "SAMPLE_GAUSS", "SAMPLE_GAMMA",
-           "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
+           "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES",
+           "MAP_CANDIDATES_2"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libsigp.so")
@@ -45,7 +46,7 @@ class InstanceDesc(C.Structure):
                 ("zp", C.c_void_p)]
 
 
-MAP_AUTO, MAP_SCENARIOS, MAP_CANDIDATES = 0, 1, 2
+MAP_AUTO, MAP_SCENARIOS, MAP_CANDIDATES, MAP_CANDIDATES_2 = 0, 1, 2, 3
 
 
 class LaunchConfig(C.Structure):

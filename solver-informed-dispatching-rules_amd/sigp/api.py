@@ -10,12 +10,14 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import MAP_AUTO, MAP_CANDIDATES, MAP_SCENARIOS, InstanceDesc, LaunchConfig, Rules, check, lib
+from ._lib import (MAP_AUTO, MAP_CANDIDATES, MAP_CANDIDATES_2, MAP_SCENARIOS, InstanceDesc,
+                   LaunchConfig, Rules, check, lib)
 from .instance import Instance
 from .rules import N_TERMS, Packed
 
 __all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info", "evaluate_multi",
-           "batch_multi", "sample_durations", "sampler", "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES"]
+           "batch_multi", "sample_durations", "sampler", "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES",
+           "MAP_CANDIDATES_2"]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -110,8 +112,9 @@ class Batch:
                                    None if stream is None else C.c_void_p(stream)))
 
     def configure(self, waves_per_item: int, mapping: int = MAP_AUTO) -> None:
-        """Launch mapping: MAP_SCENARIOS / MAP_CANDIDATES / MAP_AUTO and the
-        wavefronts per workgroup (1, 2, 4; 0 = the mapping's default)."""
+        """Launch mapping: MAP_SCENARIOS / MAP_CANDIDATES / MAP_CANDIDATES_2 /
+        MAP_AUTO and the wavefronts per workgroup (1, 2, 4; 0 = the mapping's
+        default; MAP_CANDIDATES_2 uses one)."""
         cfg = LaunchConfig(waves_per_item, mapping)
         check(lib().sigp_batch_configure(self._h, C.byref(cfg)))
 

@@ -172,7 +172,7 @@ def main(argv=None):
     ap.add_argument("--generations", type=int, default=50)
     ap.add_argument("--config", choices=["clustered", "large"], default="clustered")
     ap.add_argument("--waves", type=int, default=0, help="evaluation wavefronts per workgroup (0: automatic)")
-    ap.add_argument("--mapping", choices=["scenarios", "candidates"], default="candidates")
+    ap.add_argument("--mapping", choices=["scenarios", "candidates", "candidates2"], default="candidates")
     ap.add_argument("--pop", type=int, default=0, help="population (0: 1,024; large: 4,096)")
     ap.add_argument("--breed", type=int, default=0, help="offspring bred per generation (0: 8 x population)")
     ap.add_argument("--seed", type=int, default=1)
@@ -192,7 +192,8 @@ def main(argv=None):
     a.breed = a.breed or 8 * a.pop
     launch = None
     if a.waves:
-        launch = dict(waves_per_item=a.waves, mapping=1 if a.mapping == "scenarios" else 2)
+        launch = dict(waves_per_item=a.waves, mapping={"scenarios": 1, "candidates": 2,
+                                                       "candidates2": 3}[a.mapping])
     net, norm, stats = train_network(sc, a.rules, a.epochs, a.seed)
     _log("network", json.dumps(stats))
     gain = screening_gain(sc, net, norm, a.pool, a.seed + 100)

@@ -1,7 +1,8 @@
 """Every launch mapping gives bit-identical rewards, equal to the oracle:
 scenario lanes with 1, 2 or 4 cooperating wavefronts per 64-scenario block,
-and candidate lanes with 1, 2 or 4 wavefronts per (rule, scenario)
-simulation; the autotuner picks one of them."""
+candidate lanes with 1, 2 or 4 wavefronts per (rule, scenario) simulation,
+and segmented candidate lanes (2 simulations per wavefront); the
+autotuner picks one of them."""
 import os
 
 import numpy as np
@@ -15,7 +16,7 @@ from sigp.workloads import random_rules
 
 pytestmark = pytest.mark.gpu
 THREADS = min(16, os.cpu_count() or 1)
-MAPS = [(1, w) for w in (1, 2, 4)] + [(2, w) for w in (1, 2, 4)]   # (mapping, waves)
+MAPS = [(1, w) for w in (1, 2, 4)] + [(2, w) for w in (1, 2, 4)] + [(3, 1)]   # (mapping, waves)
 
 
 @pytest.mark.parametrize("name", ["c102", "syn"])
@@ -55,7 +56,9 @@ def test_gp_loop_shape_one_scenario_per_group(oracle):
 
 
 def test_candidate_lanes_more_candidates_than_threads(oracle):
-    """n = 288 with 256 threads: threads take two candidates each."""
+    """n = 288 with 256 threads: threads take two candidates each; in 32-lane
+    segments, up to 9 each (T = 3: the second wavefront's second segment
+    idles)."""
     from sigp.api import FactoryContext
     inst = synthetic_mixed(n=288, m=4, seed=5)
     trees = random_rules(12, EXTRA, seed=8, min_depth=1, max_depth=4)
@@ -64,10 +67,10 @@ def test_candidate_lanes_more_candidates_than_threads(oracle):
     S = gen_durations(inst.durations, 3, 11, mode="gamma", shape=2.0)
     ctx = FactoryContext(inst)
     want = oracle.evaluate(inst, pk, S, 3, mask, ctx.zp, threads=THREADS)
-    for w in (1, 2, 4):
-        print("n=288 candidate lanes waves", w, flush=True)
-        ctx.configure(w, 2)
-        assert_bitwise(ctx.evaluate(pk, S, 3, mask), want, "n=288 waves %d" % w)
+    for m, w in [(2, 1), (2, 2), (2, 4), (3, 1)]:
+        print("n=288 candidate lanes mapping", m, "waves", w, flush=True)
+        ctx.configure(w, m)
+        assert_bitwise(ctx.evaluate(pk, S, 3, mask), want, "n=288 mapping %d waves %d" % (m, w))
 
 
 def test_autotune_and_device_batch():
@@ -85,7 +88,7 @@ def test_autotune_and_device_batch():
     st = torch.from_numpy(S).to(dev)
     out = torch.zeros(64 * 128, dtype=torch.float64, device=dev)
     w, m = b.autotune(st.data_ptr(), 128)
-    assert w in (1, 2, 4) and m in (1, 2)
+    assert w in (1, 2, 4) and m in (1, 2, 3)
     b.run(st.data_ptr(), 128, out.data_ptr())
     torch.cuda.synchronize()
     assert_bitwise(out.cpu().numpy(), want, "tuned batch")

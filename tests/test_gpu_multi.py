@@ -41,7 +41,7 @@ def test_multi_instance_matches_oracle_every_mapping(oracle):
     T = 24
     blocks = [gen_durations(i.durations, T, 1000 + k, mode="gamma") for k, i in enumerate(insts)]
     want = _expected(oracle, ctxs, insts, pk, blocks, T, mask)
-    for m, w in [(0, 0), (1, 1), (1, 4), (2, 1), (2, 4)]:
+    for m, w in [(0, 0), (1, 1), (1, 4), (2, 1), (2, 4), (3, 1)]:
         print("multi mapping", m, "waves", w, flush=True)
         ctxs[0].configure(w, m)          # the batch takes its first instance's default
         got = evaluate_multi(ctxs, pk, blocks, T, mask)
