@@ -312,7 +312,9 @@ typedef struct {
     int32_t elitism;
     double crossover_rate, mutation_rate;
     int32_t T;                       /* training scenarios per run and generation */
-    sigp_launch_config launch;       /* evaluator lane mapping */
+    sigp_launch_config launch;       /* evaluator lane mapping; SIGP_MAP_AUTO: two simulations per
+                                        wavefront (SIGP_MAP_CANDIDATES_2) for n <= 128 and T >= 2,
+                                        else sigp_batch's choice */
 } sigp_gp_config;
 
 /* seeds: n_runs Mt19937 seeds; term_mask: int64[SIGP_N_TERMS] of the

@@ -913,8 +913,17 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     const int KL = std::max(1, __builtin_popcountll(rd_all));
     const int SL = std::max(1, std::max(c->max_depth, c->depth_limit));
     const bool rare = (rd_all & RARE_SLOTS) != 0;
+    // AUTO for the population's evaluation: two simulations per wavefront up
+    // to n = 128 with two or more scenarios per rule (evolved populations at
+    // n = 100 measured 6-11 % faster over a whole loop than one per
+    // wavefront), the batch's own choice otherwise
+    sigp_launch_config lc = c->launch;
+    if (lc.mapping == SIGP_MAP_AUTO && g->n <= 128 && g->T >= 2) {
+        lc.mapping = SIGP_MAP_CANDIDATES_2;
+        lc.waves_per_item = 0;
+    }
     if (status == SIGP_OK)
-        status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, c->launch, &g->bt);
+        status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, lc, &g->bt);
     if (status == SIGP_OK && screened)
         status = make_rules_batch(g, Q, (int)O, nullptr, KL, SL, rmod, rare, 1, (int)O, c->launch, &g->obt);
     if (status != SIGP_OK) {
