@@ -92,3 +92,35 @@ def test_autotune_and_device_batch():
     b.run(st.data_ptr(), 128, out.data_ptr())
     torch.cuda.synchronize()
     assert_bitwise(out.cpu().numpy(), want, "tuned batch")
+
+
+@pytest.mark.parametrize("name", ["c105", "mixed288"])
+def test_every_kernel_variant_under_every_mapping(oracle, name):
+    """All 61 slots with rmod (the RMOD and RARE variants), n = 100 and
+    n = 288 (both segmented builds), an odd T, then a one-customer and a
+    no-vehicle instance: every mapping equals the oracle."""
+    from sigp.api import FactoryContext
+    from sigp.rules import FUNCS_RMOD, SLOT_NAMES, parse_prefix
+    inst = load_instance(name) if name == "c105" else synthetic_mixed(n=288, m=4, seed=2)
+    trees = random_rules(16, SLOT_NAMES, seed=17, min_depth=2, max_depth=4, functions=FUNCS_RMOD)
+    pk = pack_trees(trees)
+    mask = np.ones(61, dtype=np.int64)
+    S = gen_durations(inst.durations, 5, 303, mode="gamma")
+    ctx = FactoryContext(inst)
+    want = oracle.evaluate(inst, pk, S, 5, mask, ctx.zp, threads=THREADS)
+    one = synthetic_uniform(n=1, m=2, seed=3)
+    S1 = gen_durations(one.durations, 3, 1)
+    c1 = FactoryContext(one)
+    p1 = pack_trees([parse_prefix("(+ SCORE NS)")])
+    want1 = oracle.evaluate(one, p1, S1, 3, term_mask_from(NS_TERMINALS), c1.zp, threads=THREADS)
+    none = synthetic_uniform(n=5, m=0, seed=3)
+    c0 = FactoryContext(none)
+    for m, w in [(1, 1), (2, 1), (3, 1)]:
+        print("variants", name, "mapping", m, flush=True)
+        for c in (ctx, c1, c0):
+            c.configure(w, m)
+        assert_bitwise(ctx.evaluate(pk, S, 5, mask), want, "%s mapping %d" % (name, m))
+        assert_bitwise(c1.evaluate(p1, S1, 3, term_mask_from(NS_TERMINALS)), want1, "n=1 mapping %d" % m)
+        out = c0.evaluate(pack_trees([parse_prefix("SCORE")]), gen_durations(none.durations, 3, 1), 3,
+                          term_mask_from(NS_TERMINALS))
+        assert np.array_equal(out, np.zeros(3)), "m=0 mapping %d" % m
