@@ -59,9 +59,23 @@ def work_model(wl):
     return doc
 
 
-def cpu_baseline(wl, zp, target_s=15.0):
+def _host_cpu():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"logical_cpus": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(wl, zp, target_s=15.0, one_core_s=4.0):
     """The reference's simulator (oracle/_ref) on host cores over a bounded
-    sample; falls back to the C restatement when the reference build is absent."""
+    sample (16 threads, then one thread on a smaller sample); falls back to
+    the C restatement when the reference build is absent."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     from oracle import Oracle, Reference
     from sigp.rules import pack_trees
@@ -71,24 +85,31 @@ def cpu_baseline(wl, zp, target_s=15.0):
     else:
         impl, kind = Oracle(), "port"
 
-    def run(nr, ns):
+    def run(nr, ns, th):
         step = max(1, wl.packed.n_rules // nr)
         trees = [wl.trees[i] for i in range(0, wl.packed.n_rules, step)][:nr]
         pk = pack_trees(trees)
         S = np.ascontiguousarray(wl.sampled[:ns])
         t0 = time.perf_counter()
-        impl.evaluate(wl.inst, pk, S, ns, wl.mask, zp, threads=threads)
+        impl.evaluate(wl.inst, pk, S, ns, wl.mask, zp, threads=th)
         return time.perf_counter() - t0, pk.n_rules * ns
 
-    dt, sims = run(threads, 4)
+    dt, sims = run(threads, 4, threads)
     rate = sims / max(dt, 1e-6)
     want = int(rate * target_s)
     nr = max(threads, min(wl.packed.n_rules, want // 32))
     ns = max(1, min(wl.n_scen, want // nr))
-    dt, sims = run(nr, ns)
+    dt, sims = run(nr, ns, threads)
+    # one thread: every rule of the sample above (the rule mix matters more
+    # than the scenario count), fewer scenarios
+    ns1 = max(1, min(ns, int(sims / dt / threads * one_core_s) // nr))
+    dt1, sims1 = run(nr, ns1, 1)
     return {"value": sims / dt, "unit": "sims/s", "cores": threads, "kind": kind,
             "sample": "%d rules (every %d-th) x first %d scenarios of the workload, %.1f s"
-                      % (nr, max(1, wl.packed.n_rules // nr), ns, dt)}
+                      % (nr, max(1, wl.packed.n_rules // nr), ns, dt),
+            "value_1core": sims1 / dt1,
+            "sample_1core": "the same rules x first %d scenarios, %.1f s" % (ns1, dt1),
+            "host": _host_cpu()}
 
 
 def describe(kind, wl):
