@@ -291,6 +291,8 @@ def main():
     ap.add_argument("--mapping", choices=sorted(MAPPINGS), default="scenarios",
                     help="lane mapping used with --waves (candidates2: 2 simulations per wavefront, "
                          "--waves 1)")
+    ap.add_argument("--no-scenario-order", action="store_true",
+                    help="scenario lanes in index order (sigp_batch_order_scenarios off)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after timing, run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
@@ -337,6 +339,8 @@ def main():
         n_rows = wl.sampled.shape[0]
         host_rows = wl.sampled
         P = wl.packed.n_rules
+    ordered = batch.order_scenarios(not args.no_scenario_order)
+    log("scenario order for scenario lanes:", "on" if ordered else "off")
     sampled = torch.from_numpy(host_rows).to(dev)
     pairs = torch.zeros(P * T, dtype=torch.float64, device=dev)
     means = torch.zeros(P, dtype=torch.float64, device=dev)

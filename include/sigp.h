@@ -167,6 +167,18 @@ typedef struct {
     int32_t mapping;          /* SIGP_MAP_* */
 } sigp_launch_config;
 
+/* Scenario order of scenario lanes (no counterpart in the reference; results
+ * are identical either way).  A 64-lane block of scenario lanes simulates 64
+ * scenarios of one rule in lockstep; the first decision is the same in every
+ * scenario, and the first service time sets the clock after it, so blocks
+ * whose scenarios are adjacent by the duration of the rule's first customer
+ * keep their routes together longer.  Enabled by default where the batch
+ * qualifies (64 <= T <= 4096, orders within 256 MiB): each pair's first
+ * customer is found at creation and sigp_batch_run sorts the rows on its
+ * stream before the simulation.  enable = 0 turns it off; *active (nullable)
+ * reports whether it is on. */
+sigp_status sigp_batch_order_scenarios(sigp_batch *batch, int32_t enable, int32_t *active);
+
 /* Mapping of one batch. */
 sigp_status sigp_batch_configure(sigp_batch *batch, const sigp_launch_config *cfg);
 

@@ -238,6 +238,13 @@ struct ScenSpec {
     int T;                   // scenarios per rule
     int group_size;          // rule p reads rows (p / group_size) * T + s of its instance
     int nblk;                // ceil(T / 64)
+    // Scenario order of scenario lanes (null: index order): lane j of pair
+    // (k, p)'s scenario blocks simulates scenario perm[perm_off + (g * n +
+    // c) * T + j], g = p / group_size, c = first[k * P + p] (the rule's first
+    // customer; -1: index order).  The result still goes to slot (p, s).
+    const int32_t *perm;
+    int32_t *first;
+    int first_only;          // candidate lanes: only record each pair's first decision in `first`
 };
 
 // One instance of a launch: its tables and where its scenario rows and
@@ -248,6 +255,7 @@ struct InstSlot {
     FusedTabs tabs;
     long long samp_off;   // first double of the instance's scenario rows
     long long out_off;    // first result: instance k's pairs are out[out_off + p * T + s]
+    long long perm_off;   // first entry of the instance's scenario orders
 };
 // Kernels take the slots as a read-only, non-aliased argument, so that their
 // wave-uniform reads of it are scalar loads.
@@ -546,14 +554,22 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const int s = blk * 64 + lane;
+    int s = blk * 64 + lane;
     const bool active = s < S.T;
+    const int g = p / S.group_size;
+    if (S.perm && active) {
+        // lanes of a block take scenarios adjacent in the rule's order, so
+        // their routes part later (the first decision is the same in every
+        // scenario; the first service time sets the clock after it)
+        const int c1 = S.first[pair];
+        if (c1 >= 0) s = S.perm[slots[k].perm_off + ((size_t)g * I.n + c1) * S.T + s];
+    }
     const int off = R.off[p], L = R.len[p];
     const cip ops = R.ops + off;
     const cip args = R.args + 2 * (size_t)off;
     const uint64_t rd = R.rd[p];
     const uint64_t fl = R.flags[p];
-    const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
+    const size_t row = (size_t)g * S.T + (active ? s : 0);
     const double *__restrict__ samp = S.sampled + slots[k].samp_off + row * (size_t)I.n;
     double *tv = lds_rows + (size_t)wv * (R.KL + R.SL) * 64 + lane;
     double *st = tv + (size_t)R.KL * 64;
@@ -674,12 +690,20 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
     int cur = 0, veh = 0;
     double ct = 0.0, total = 0.0;
     bool running = I.m > 0;
+    if (S.first_only && !running) {
+        if (threadIdx.x == 0) S.first[pair] = -1;
+        return;
+    }
     while (running) {
         double bp = -1e300;
         const int rem = I.m - veh - 1;
         int best = decide<NW, RMOD, RARE, true>(I, F, ops, args, L, rd, fl, cur, ct, rem, vis,
                                                 (int)threadIdx.x, (int)blockDim.x, tv, st, bp);
         best = block_argmax(bp, best, W, wv, lane, rpri, ridx);
+        if (S.first_only) {   // uniform: every thread holds the winner
+            if (threadIdx.x == 0) S.first[pair] = best;
+            return;
+        }
         if (best < 0) {
             ++veh;
             cur = 0;
@@ -711,6 +735,32 @@ __device__ __forceinline__ int seg_argmax(double pri, int idx) {
         if (beats(op, oi, pri, idx)) { pri = op; idx = oi; }
     }
     return idx;
+}
+
+// Scenario order for scenario lanes: workgroup b = g * n + c ranks group g's
+// T scenarios by their duration of customer c (a total order on the bit
+// patterns, ties by index, so the result is always a permutation) and
+// writes perm[b * T + rank] = s.  T <= ORDER_MAX_T.
+constexpr int ORDER_MAX_T = 4096;
+__global__ __launch_bounds__(256) void order_rows_kernel(const double *__restrict__ sampled, int n, int T,
+                                                         int32_t *__restrict__ perm) {
+    __shared__ unsigned long long key[ORDER_MAX_T];
+    const int b = (int)blockIdx.x;
+    const int g = b / n, c = b - g * n;
+    for (int s = threadIdx.x; s < T; s += blockDim.x) {
+        const unsigned long long u = (unsigned long long)__double_as_longlong(sampled[((size_t)g * T + s) * n + c]);
+        key[s] = (u >> 63) ? ~u : (u | (1ull << 63));
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < T; s += blockDim.x) {
+        const unsigned long long ks = key[s];
+        int r = 0;
+        for (int j = 0; j < T; ++j) {
+            const unsigned long long kj = key[j];
+            r += (kj < ks || (kj == ks && j < s)) ? 1 : 0;
+        }
+        perm[(size_t)b * T + r] = s;
+    }
 }
 
 // Segmented candidate lanes: one wavefront simulates 64 / SEGW scenarios of
@@ -921,6 +971,13 @@ struct sigp_batch {
     double *pairs_tmp = nullptr;    // used when the caller wants means only
     hipStream_t side[2] = {nullptr, nullptr};   // size classes 2 and 3 run here, concurrently
     hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
+    // scenario order of scenario lanes (ScenSpec::perm): each pair's first
+    // customer, computed once, and per run the order of every group's rows
+    // by each customer's duration; null when the batch does not qualify
+    int32_t *first = nullptr;
+    int32_t *perm = nullptr;
+    bool order_on = false;
+    int G = 1;                      // scenario groups per instance
 };
 
 static sigp_status check_device(int *dev_out) {
@@ -1259,8 +1316,12 @@ static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, 
 // A batch over `insts` (one instance, or several evaluating the same rules):
 // the rule arrays, one InstSlot per instance and, per size class, the
 // longest-first order of its (instance, rule) pairs, cost = rule cost x n^2.
+static sigp_status setup_order(sigp_batch *bt, long long perms);
+static constexpr long long ORDER_MAX_BYTES = 256LL << 20;
+
 static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool multi,
-                              const PreparedRules &pr, int T, int group_size, sigp_batch **out) {
+                              const PreparedRules &pr, int T, int group_size, sigp_batch **out,
+                              bool order = true) {
     const int P = pr.P, K = (int)insts.size();
     if ((long long)K * P > 0x7fffffffLL) return fail(SIGP_INVALID_ARG, "instances x rules exceeds 2^31");
     sigp_batch *bt = new sigp_batch();
@@ -1277,13 +1338,16 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
     bt->has_rare = (pr.any_rd & RARE_SLOTS) != 0;
 
     std::vector<InstSlot> slots(K);
-    long long samp = 0;
+    long long samp = 0, perms = 0;
+    bt->G = multi || P == 0 ? 1 : (P - 1) / group_size + 1;
     for (int k = 0; k < K; ++k) {
         slots[k].dev = insts[k]->dev;
         slots[k].tabs = insts[k]->tabs;
         slots[k].samp_off = samp;
         slots[k].out_off = (long long)k * P * T;
+        slots[k].perm_off = perms;
         if (multi) samp += (long long)T * insts[k]->n;
+        perms += (long long)bt->G * insts[k]->n * T;
     }
     const int classes[3] = {4, 9, 16};
     std::vector<int32_t> orders[3], samples[3];
@@ -1378,6 +1442,13 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
         bt->units[u].order = (cip)(base + o_ord[u]);
         bt->units[u].tune_order = (cip)(base + o_smp[u]);
     }
+    if (order && P > 0 && T >= 64 && T <= ORDER_MAX_T && perms * 4 <= ORDER_MAX_BYTES) {
+        const sigp_status st = setup_order(bt, perms);
+        if (st != SIGP_OK) {
+            sigp_batch_destroy(bt);
+            return st;
+        }
+    }
     *out = bt;
     return SIGP_OK;
 }
@@ -1421,6 +1492,8 @@ extern "C" sigp_status sigp_batch_destroy(sigp_batch *bt) {
     if (bt->fork) (void)hipEventDestroy(bt->fork);
     if (bt->dbuf) (void)hipFree(bt->dbuf);
     if (bt->pairs_tmp) (void)hipFree(bt->pairs_tmp);
+    if (bt->first) (void)hipFree(bt->first);
+    if (bt->perm) (void)hipFree(bt->perm);
     delete bt;
     return SIGP_OK;
 }
@@ -1431,6 +1504,13 @@ static sigp_status check_config(const sigp_launch_config *cfg) {
         return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
     if (cfg->mapping < SIGP_MAP_AUTO || cfg->mapping > SIGP_MAP_CANDIDATES_2)
         return fail(SIGP_INVALID_ARG, "mapping must be SIGP_MAP_AUTO, _SCENARIOS, _CANDIDATES or _CANDIDATES_2");
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_batch_order_scenarios(sigp_batch *bt, int32_t enable, int32_t *active) {
+    if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
+    bt->order_on = enable != 0 && bt->perm != nullptr;
+    if (active) *active = bt->order_on ? 1 : 0;
     return SIGP_OK;
 }
 
@@ -1529,14 +1609,63 @@ static void launch_class(const sigp_batch *bt, const LaunchUnit &u, int mapping,
     }
 }
 
+// Each pair's first customer (the first decision reads no sampled duration,
+// so it is the same in every scenario): candidate lanes in first-only mode,
+// one workgroup per pair, at batch creation.
+static sigp_status setup_order(sigp_batch *bt, long long perms) {
+    const size_t pairs = (size_t)bt->insts.size() * bt->P;
+    HIP_TRY(hipMalloc((void **)&bt->first, pairs * 4));
+    HIP_TRY(hipMalloc((void **)&bt->perm, (size_t)perms * 4));
+    for (int u = 0; u < bt->n_units; ++u) {
+        const LaunchUnit &lu = bt->units[u];
+        ScenSpec S{};
+        S.T = 1;
+        S.group_size = bt->group_size;
+        S.nblk = 1;
+        S.first = bt->first;
+        S.first_only = 1;
+        if (lu.nwb == 4) launch_class<4>(bt, lu, SIGP_MAP_CANDIDATES, 1, (unsigned)lu.n_pairs, S, nullptr, nullptr);
+        else if (lu.nwb == 9) launch_class<9>(bt, lu, SIGP_MAP_CANDIDATES, 1, (unsigned)lu.n_pairs, S, nullptr, nullptr);
+        else launch_class<16>(bt, lu, SIGP_MAP_CANDIDATES, 1, (unsigned)lu.n_pairs, S, nullptr, nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    bt->order_on = true;
+    return SIGP_OK;
+}
+
+static bool uses_scenario_lanes(const sigp_batch *bt) {
+    for (int u = 0; u < bt->n_units; ++u)
+        if (resolved_mapping(bt, bt->units[u]) == SIGP_MAP_SCENARIOS) return true;
+    return false;
+}
+
+// Per run: the order of each instance's scenario groups by each customer's
+// duration (order_rows_kernel), on `stream` before the simulation.
+static sigp_status order_scenarios(const sigp_batch *bt, const double *sampled, hipStream_t stream) {
+    long long samp = 0, perms = 0;
+    for (const sigp_instance *inst : bt->insts) {
+        const int n = inst->n;
+        if (n > 0) order_rows_kernel<<<(unsigned)(bt->G * n), 256, 0, stream>>>(sampled + samp, n, bt->T, bt->perm + perms);
+        if (bt->multi) samp += (long long)bt->T * n;
+        perms += (long long)bt->G * n * bt->T;
+    }
+    HIP_TRY(hipGetLastError());
+    return SIGP_OK;
+}
+
 static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const double *sampled,
                                double *out, hipStream_t stream) {
-    ScenSpec S;
+    ScenSpec S{};
     S.sampled = sampled;
     S.T = bt->T;
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
     int mapping = resolved_mapping(bt, u);
+    if (bt->order_on && mapping == SIGP_MAP_SCENARIOS) {
+        S.perm = bt->perm;
+        S.first = bt->first;
+    }
     if (u.nwb > 9 && mapping == SIGP_MAP_CANDIDATES_2)
         mapping = SIGP_MAP_CANDIDATES;   // segments are built for n <= 288 only
     const int w = resolved_waves(bt, u, mapping);
@@ -1570,6 +1699,10 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
         if (!out_mean_dev) return SIGP_OK;
         if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->n_out * 8));
         pairs = bt->pairs_tmp;
+    }
+    if (bt->order_on && uses_scenario_lanes(bt)) {
+        const sigp_status st = order_scenarios(bt, sampled_dev, s);
+        if (st != SIGP_OK) return st;
     }
     if (bt->n_units == 1) {
         const sigp_status st = launch_unit(bt, bt->units[0], sampled_dev, pairs, s);
@@ -1648,6 +1781,10 @@ extern "C" sigp_status sigp_batch_autotune(sigp_batch *bt, const double *sampled
     if (n_rows < bt->rows_needed) return fail(SIGP_INVALID_ARG, "sampled has too few rows");
     if (!bt->pairs_tmp) HIP_TRY(hipMalloc((void **)&bt->pairs_tmp, (size_t)bt->n_out * 8));
     hipStream_t s = (hipStream_t)stream;
+    if (bt->order_on) {
+        const sigp_status st = order_scenarios(bt, sampled_dev, s);
+        if (st != SIGP_OK) return st;
+    }
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {

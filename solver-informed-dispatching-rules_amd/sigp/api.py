@@ -118,6 +118,13 @@ class Batch:
         cfg = LaunchConfig(waves_per_item, mapping)
         check(lib().sigp_batch_configure(self._h, C.byref(cfg)))
 
+    def order_scenarios(self, enable: bool) -> bool:
+        """Scenario order of scenario lanes (sigp_batch_order_scenarios); returns
+        whether it is on (it needs 64 <= T <= 4096)."""
+        on = C.c_int32(0)
+        check(lib().sigp_batch_order_scenarios(self._h, 1 if enable else 0, C.byref(on)))
+        return bool(on.value)
+
     def autotune(self, sampled_ptr: int, n_rows: int, stream: Optional[int] = None):
         """Time every mapping, keep the fastest; returns (waves_per_item, mapping)."""
         cfg = LaunchConfig(0, 0)

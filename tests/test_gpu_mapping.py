@@ -124,3 +124,55 @@ def test_every_kernel_variant_under_every_mapping(oracle, name):
         out = c0.evaluate(pack_trees([parse_prefix("SCORE")]), gen_durations(none.durations, 3, 1), 3,
                           term_mask_from(NS_TERMINALS))
         assert np.array_equal(out, np.zeros(3)), "m=0 mapping %d" % m
+
+
+def test_scenario_order_changes_nothing(oracle):
+    """Scenario lanes with and without the scenario order
+    (sigp_batch_order_scenarios): identical, and equal to the oracle, on one
+    instance (T = 130: a partial block), on grouped rows (3 groups) and on a
+    multi-instance batch."""
+    import torch
+    from sigp.api import FactoryContext, batch_multi
+    dev = torch.device("cuda:0")
+    inst = load_instance("r102")
+    trees = random_rules(24, NS_TERMINALS + EXTRA, seed=29, min_depth=2, max_depth=5)
+    pk = pack_trees(trees)
+    mask = term_mask_from(NS_TERMINALS + EXTRA)
+    ctx = FactoryContext(inst)
+    ctx.configure(4, 1)
+
+    def run(b, S, n_out, n_rows=None):
+        st = torch.from_numpy(np.ascontiguousarray(S)).to(dev)
+        out = torch.zeros(n_out, dtype=torch.float64, device=dev)
+        b.run(st.data_ptr(), S.shape[0] if n_rows is None else n_rows, out.data_ptr())
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    T = 130
+    S = gen_durations(inst.durations, T, 4242, mode="gamma")
+    want = oracle.evaluate(inst, pk, S, T, mask, ctx.zp, threads=THREADS)
+    b = ctx.batch(pk, T, mask)
+    assert b.order_scenarios(True)
+    on = run(b, S, 24 * T)
+    assert not b.order_scenarios(False)
+    off = run(b, S, 24 * T)
+    assert_bitwise(on, want, "ordered")
+    assert_bitwise(off, want, "index order")
+
+    G, gs, T = 3, 8, 70
+    S = np.concatenate([gen_durations(inst.durations, T, 500 + g) for g in range(G)])
+    want = oracle.evaluate(inst, pk, S, T, mask, ctx.zp, group_size=gs, threads=THREADS)
+    assert_bitwise(ctx.evaluate_grouped(pk, S, G, T, gs, mask), want, "grouped, ordered")
+
+    insts = [load_instance("c101"), synthetic_mixed(n=150, m=3, seed=4), synthetic_uniform(n=60, m=2, seed=6)]
+    ctxs = [FactoryContext(i) for i in insts]
+    T = 64
+    blocks = [gen_durations(i.durations, T, 77 + k, mode="gamma") for k, i in enumerate(insts)]
+    want = np.concatenate([oracle.evaluate(i, pk, B, T, mask, c.zp, threads=THREADS)
+                           for i, c, B in zip(insts, ctxs, blocks)])
+    bm = batch_multi(ctxs, pk, T, mask)
+    bm.configure(4, 1)
+    assert bm.order_scenarios(True)
+    flat = np.concatenate([B.ravel() for B in blocks])   # instance blocks back to back
+    assert_bitwise(run(bm, flat, len(insts) * 24 * T, len(insts) * T), want, "multi, ordered")
+    ctx.configure(0, 0)
