@@ -245,7 +245,15 @@ struct ScenSpec {
     const int32_t *perm;
     int32_t *first;
     int first_only;          // candidate lanes: only record each pair's first decision in `first`
+    long long n_ranks;       // scenario lanes: pairs of the launch (the grid is padded, see sim_kernel)
 };
+
+// Workgroups are dispatched to the 8 XCDs round-robin by index.  Scenario
+// lanes give XCD x the pairs x, x + 8, ... of the launch order, each with all
+// its scenario blocks in a row, so a rule's blocks share one L2 (its rewards'
+// cache lines, written in scenario order or permuted, merge there) and the
+// longest-first order holds on every XCD.
+constexpr int N_XCD = 8;
 
 // One instance of a launch: its tables and where its scenario rows and
 // results start.  A single-instance batch has one slot; a multi-instance
@@ -544,8 +552,11 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     __shared__ double lpri[4][64];
     __shared__ int lidx[4][64];
     extern __shared__ double lds_rows[];
-    const int rank = (int)blockIdx.x / S.nblk;
-    const int blk = (int)blockIdx.x - rank * S.nblk;
+    const int xcd = (int)(blockIdx.x % N_XCD);
+    const int local = (int)(blockIdx.x / N_XCD);
+    const int rank = xcd + N_XCD * (local / S.nblk);
+    const int blk = local % S.nblk;
+    if (rank >= S.n_ranks) return;   // padding of the last round (whole workgroup)
     const int pair = R.order ? R.order[rank] : rank;
     const int k = pair / R.P;
     const int p = pair - k * R.P;
@@ -951,6 +962,8 @@ struct LaunchUnit {
     int waves = 0;                  // tuned for this class; 0 = follow the batch
     cip tune_order = nullptr;       // every s-th pair of `order`: the autotuner's sample
     long long tune_pairs = 0;
+    cip check_order = nullptr;      // a 4x larger sample: the autotuner's check of its ranking
+    long long check_pairs = 0;
 };
 
 struct sigp_batch {
@@ -1350,9 +1363,11 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
         perms += (long long)bt->G * insts[k]->n * T;
     }
     const int classes[3] = {4, 9, 16};
-    std::vector<int32_t> orders[3], samples[3];
-    // the autotuner times each mapping on about 2^19 simulations of a class
+    std::vector<int32_t> orders[3], samples[3], checks[3];
+    // the autotuner times each mapping on about 2^19 simulations of a class,
+    // and checks a close ranking on about 2^21
     const long long tune_cap = std::max(1LL, (1LL << 19) / std::max(T, 1));
+    const long long check_cap = 4 * tune_cap;
     for (int c = 0; c < 3; ++c) {
         LaunchUnit &u = bt->units[bt->n_units];
         std::vector<std::pair<float, int32_t>> items;
@@ -1370,9 +1385,12 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
         for (const auto &it : items) orders[bt->n_units].push_back(it.second);
         const long long stride = ((long long)items.size() + tune_cap - 1) / tune_cap;
         for (size_t i = 0; i < items.size(); i += (size_t)stride) samples[bt->n_units].push_back(items[i].second);
+        const long long stride4 = ((long long)items.size() + check_cap - 1) / check_cap;
+        for (size_t i = 0; i < items.size(); i += (size_t)stride4) checks[bt->n_units].push_back(items[i].second);
         u.nwb = classes[c];
         u.n_pairs = (long long)items.size();
         u.tune_pairs = (long long)samples[bt->n_units].size();
+        u.check_pairs = (long long)checks[bt->n_units].size();
         ++bt->n_units;
     }
 
@@ -1384,16 +1402,18 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
         };
         std::stable_sort(idx, idx + bt->n_units, [&](int a, int b) { return work(a) > work(b); });
         LaunchUnit us[3];
-        std::vector<int32_t> os[3], ss[3];
+        std::vector<int32_t> os[3], ss[3], cs[3];
         for (int u = 0; u < bt->n_units; ++u) {
             us[u] = bt->units[idx[u]];
             os[u].swap(orders[idx[u]]);
             ss[u].swap(samples[idx[u]]);
+            cs[u].swap(checks[idx[u]]);
         }
         for (int u = 0; u < bt->n_units; ++u) {
             bt->units[u] = us[u];
             orders[u].swap(os[u]);
             samples[u].swap(ss[u]);
+            checks[u].swap(cs[u]);
         }
     }
     size_t off = 0;
@@ -1401,10 +1421,11 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
     const size_t o_ops = take(pr.ops.size() * 4), o_args = take(pr.args.size() * 4),
                  o_off = take((size_t)P * 4), o_len = take((size_t)P * 4), o_rd = take((size_t)P * 8),
                  o_fl = take((size_t)P * 8), o_slots = take((size_t)K * sizeof(InstSlot));
-    size_t o_ord[3], o_smp[3];
+    size_t o_ord[3], o_smp[3], o_chk[3];
     for (int u = 0; u < bt->n_units; ++u) {
         o_ord[u] = take(orders[u].size() * 4);
         o_smp[u] = take(samples[u].size() * 4);
+        o_chk[u] = take(checks[u].size() * 4);
     }
     char *base = nullptr;
     if (hipMalloc((void **)&base, off) != hipSuccess) {
@@ -1421,7 +1442,8 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
               up(o_slots, slots.data(), (size_t)K * sizeof(InstSlot));
     for (int u = 0; ok && u < bt->n_units; ++u)
         ok = up(o_ord[u], orders[u].data(), orders[u].size() * 4) &&
-             up(o_smp[u], samples[u].data(), samples[u].size() * 4);
+             up(o_smp[u], samples[u].data(), samples[u].size() * 4) &&
+             up(o_chk[u], checks[u].data(), checks[u].size() * 4);
     if (!ok) {
         (void)hipFree(base);
         delete bt;
@@ -1441,6 +1463,7 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
     for (int u = 0; u < bt->n_units; ++u) {
         bt->units[u].order = (cip)(base + o_ord[u]);
         bt->units[u].tune_order = (cip)(base + o_smp[u]);
+        bt->units[u].check_order = (cip)(base + o_chk[u]);
     }
     if (order && P > 0 && T >= 64 && T <= ORDER_MAX_T && perms * 4 <= ORDER_MAX_BYTES) {
         const sigp_status st = setup_order(bt, perms);
@@ -1669,10 +1692,11 @@ static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const 
     if (u.nwb > 9 && mapping == SIGP_MAP_CANDIDATES_2)
         mapping = SIGP_MAP_CANDIDATES;   // segments are built for n <= 288 only
     const int w = resolved_waves(bt, u, mapping);
+    S.n_ranks = u.n_pairs;
     const long long items =
-        u.n_pairs * (mapping == SIGP_MAP_CANDIDATES    ? (long long)S.T
-                     : mapping == SIGP_MAP_SCENARIOS   ? (long long)S.nblk
-                                                       : (long long)((S.T + 1) / 2));
+        mapping == SIGP_MAP_CANDIDATES   ? u.n_pairs * (long long)S.T
+        : mapping == SIGP_MAP_SCENARIOS  ? (u.n_pairs + N_XCD - 1) / N_XCD * N_XCD * (long long)S.nblk
+                                         : u.n_pairs * (long long)((S.T + 1) / 2);
     if (items > 0xffffffffLL) return fail(SIGP_INVALID_ARG, "launch exceeds 2^32 workgroups");
     if (u.nwb == 4) launch_class<4>(bt, u, mapping, w, (unsigned)items, S, out, stream);
     else if (u.nwb == 9) launch_class<9>(bt, u, mapping, w, (unsigned)items, S, out, stream);
@@ -1748,6 +1772,8 @@ static sigp_status tune_unit(sigp_batch *bt, LaunchUnit &u, const double *sample
     const int groups = (u.max_n + 63) / 64;
     int best_w = 0, best_m = SIGP_MAP_AUTO;
     float best_ms = 0.0f;
+    float map_ms[4] = {0.0f, 0.0f, 0.0f, 0.0f};   // best time of each mapping (SIGP_MAP_* 1..3)
+    int map_w[4] = {0, 0, 0, 0};
     for (const Cand &c : cands) {
         if (fit_waves(bt, c.w) != c.w) continue;                           // LDS does not fit
         if (c.mapping == SIGP_MAP_CANDIDATES_2 && u.nwb > 9) continue;      // built for n <= 288
@@ -1764,6 +1790,41 @@ static sigp_status tune_unit(sigp_batch *bt, LaunchUnit &u, const double *sample
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         }
         if (best_w == 0 || ms < best_ms) { best_ms = ms; best_w = c.w; best_m = c.mapping; }
+        if (map_w[c.mapping] == 0 || ms < map_ms[c.mapping]) { map_ms[c.mapping] = ms; map_w[c.mapping] = c.w; }
+    }
+    // The sample can rank scenario lanes and candidate lanes the wrong way
+    // round when they are close: a small sample is bound by its heaviest
+    // rules' latency, which candidate lanes (a workgroup per scenario) hide
+    // better, a whole class by throughput.  (The 40-instance workload's
+    // n <= 288 class timed candidate lanes 12 % ahead on 2^19 simulations and
+    // 35 % behind on the whole batch.)  So the best of each family, within
+    // 25 % of each other, are timed once more on a 4x larger sample.
+    {
+        const int cand = map_w[SIGP_MAP_CANDIDATES_2] &&
+                                 (!map_w[SIGP_MAP_CANDIDATES] || map_ms[SIGP_MAP_CANDIDATES_2] < map_ms[SIGP_MAP_CANDIDATES])
+                             ? SIGP_MAP_CANDIDATES_2 : SIGP_MAP_CANDIDATES;
+        const int scen = SIGP_MAP_SCENARIOS;
+        if (u.check_pairs > u.tune_pairs && map_w[scen] && map_w[cand] &&
+            std::max(map_ms[scen], map_ms[cand]) < 1.25f * std::min(map_ms[scen], map_ms[cand])) {
+            LaunchUnit chk = u;
+            chk.order = u.check_order;
+            chk.n_pairs = u.check_pairs;
+            const int ms_m[2] = {scen, cand};
+            float t[2] = {0.0f, 0.0f};
+            for (int i = 0; i < 2; ++i) {
+                chk.mapping = ms_m[i];
+                chk.waves = map_w[ms_m[i]];
+                HIP_TRY(hipEventRecord(e0, s));
+                const sigp_status st = launch_unit(bt, chk, sampled, bt->pairs_tmp, s);
+                if (st != SIGP_OK) return st;
+                HIP_TRY(hipEventRecord(e1, s));
+                HIP_TRY(hipEventSynchronize(e1));
+                HIP_TRY(hipEventElapsedTime(&t[i], e0, e1));
+            }
+            const int win = t[0] <= t[1] ? scen : cand;
+            best_m = win;
+            best_w = map_w[win];
+        }
     }
     u.mapping = best_m;
     u.waves = best_w;
