@@ -182,13 +182,16 @@ sigp_status sigp_batch_order_scenarios(sigp_batch *batch, int32_t enable, int32_
 /* Mapping of one batch. */
 sigp_status sigp_batch_configure(sigp_batch *batch, const sigp_launch_config *cfg);
 
-/* Times every candidate configuration (both mappings, 1, 2 and 4 wavefronts
+/* Times every candidate configuration (every mapping, 1, 2 and 4 wavefronts
  * where they fit), twice each, on `stream` (device pointers as in
  * sigp_batch_run) and keeps the fastest by the second run, per size class.
  * Each class is timed on an evenly strided sample of its (instance, rule)
  * pairs, about 2^19 simulations, so tuning costs a fraction of a run of a
- * large batch.  cfg_out (nullable) receives the configuration of the class
- * with the most work.  Synchronous; the results it writes are scratch. */
+ * large batch; when the best scenario-lane and candidate-lane
+ * configurations come within 25 % of each other, the two are timed once
+ * more on a 4x larger sample, which decides.  cfg_out (nullable) receives
+ * the configuration of the class with the most work.  Synchronous; the
+ * results it writes are scratch. */
 sigp_status sigp_batch_autotune(sigp_batch *batch, const double *sampled_dev, int64_t n_rows,
                                 void *stream, sigp_launch_config *cfg_out);
 
