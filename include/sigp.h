@@ -172,7 +172,8 @@ typedef struct {
  * scenarios of one rule in lockstep; the first decision is the same in every
  * scenario, and the first service time sets the clock after it, so blocks
  * whose scenarios are adjacent by the duration of the rule's first customer
- * keep their routes together longer.  Enabled by default where the batch
+ * keep their routes together longer (so do the two simulations of a
+ * SIGP_MAP_CANDIDATES_2 wavefront).  Enabled by default where the batch
  * qualifies (64 <= T <= 4096, orders within 256 MiB): each pair's first
  * customer is found at creation and sigp_batch_run sorts the rows on its
  * stream before the simulation.  enable = 0 turns it off; *active (nullable)

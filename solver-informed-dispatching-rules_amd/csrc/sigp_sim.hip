@@ -796,14 +796,19 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
     const FusedTabs F = slots[k].tabs;
     const int lane = threadIdx.x & 63;
     const int sl = lane % SEGW;
-    const int s = blk * SEGS + lane / SEGW;
+    int s = blk * SEGS + lane / SEGW;
     const bool active = s < S.T;
+    const int g = p / S.group_size;
+    if (S.perm && active) {   // scenario order, as in sim_kernel
+        const int c1 = S.first[pair];
+        if (c1 >= 0) s = S.perm[slots[k].perm_off + ((size_t)g * I.n + c1) * S.T + s];
+    }
     const int off = R.off[p], L = R.len[p];
     const cip ops = R.ops + off;
     const cip args = R.args + 2 * (size_t)off;
     const uint64_t rd = R.rd[p];
     const uint64_t fl = R.flags[p];
-    const size_t row = (size_t)(p / S.group_size) * S.T + (active ? s : 0);
+    const size_t row = (size_t)g * S.T + (active ? s : 0);
     const double *__restrict__ samp = S.sampled + slots[k].samp_off + row * (size_t)I.n;
     double *tv = lds_rows + lane;
     double *st = tv + (size_t)R.KL * 64;
@@ -1658,8 +1663,10 @@ static sigp_status setup_order(sigp_batch *bt, long long perms) {
 }
 
 static bool uses_scenario_lanes(const sigp_batch *bt) {
-    for (int u = 0; u < bt->n_units; ++u)
-        if (resolved_mapping(bt, bt->units[u]) == SIGP_MAP_SCENARIOS) return true;
+    for (int u = 0; u < bt->n_units; ++u) {
+        const int m = resolved_mapping(bt, bt->units[u]);
+        if (m == SIGP_MAP_SCENARIOS || (m == SIGP_MAP_CANDIDATES_2 && bt->units[u].nwb <= 9)) return true;
+    }
     return false;
 }
 
@@ -1685,7 +1692,7 @@ static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const 
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
     int mapping = resolved_mapping(bt, u);
-    if (bt->order_on && mapping == SIGP_MAP_SCENARIOS) {
+    if (bt->order_on && (mapping == SIGP_MAP_SCENARIOS || mapping == SIGP_MAP_CANDIDATES_2)) {
         S.perm = bt->perm;
         S.first = bt->first;
     }
