@@ -640,6 +640,55 @@ __global__ __launch_bounds__(256) void order_kernel(const float *cost, int P, in
     if (i < P) order[rank] = i;
 }
 
+// Repeated trees.  Trees of one run with the same tokens simulate to the
+// same rewards on the run's scenarios (the simulation is deterministic), so
+// each is simulated once: tree_hash_kernel hashes every parent (FNV-1a of
+// its length and tokens), tree_rep_kernel gives each the first tree of its
+// run with the same tokens (compared in full, so a hash collision cannot
+// merge two trees) and a launch-order cost of -1 if that is another tree
+// (repeats go last in the order and their workgroups exit at once), and
+// dup_fill_kernel copies the rewards and fitness of that first tree after
+// the evaluation.
+__global__ __launch_bounds__(256) void tree_hash_kernel(const int16_t *tok, const int32_t *len, int P,
+                                                        uint64_t *h) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= P) return;
+    const int L = len[i];
+    const int16_t *t = tok + (size_t)i * MAXL;
+    uint64_t x = 1469598103934665603ull;
+    x = (x ^ (uint64_t)(uint32_t)L) * 1099511628211ull;
+    for (int k = 0; k < L; ++k) x = (x ^ (uint64_t)(uint16_t)t[k]) * 1099511628211ull;
+    h[i] = x;
+}
+
+__global__ __launch_bounds__(256) void tree_rep_kernel(const int16_t *tok, const int32_t *len, const uint64_t *h,
+                                                       const float *cost, int P, int pop, int32_t *rep,
+                                                       float *dcost) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= P) return;
+    const int L = len[i];
+    const uint64_t hi = h[i];
+    const int16_t *a = tok + (size_t)i * MAXL;
+    int r = i;
+    for (int j = i - i % pop; j < i && r == i; ++j) {
+        if (h[j] != hi || len[j] != L) continue;
+        const int16_t *b = tok + (size_t)j * MAXL;
+        int k = 0;
+        while (k < L && a[k] == b[k]) ++k;
+        if (k == L) r = j;
+    }
+    rep[i] = r;
+    dcost[i] = r == i ? cost[i] : -1.0f;
+}
+
+__global__ __launch_bounds__(256) void dup_fill_kernel(const int32_t *rep, int T, double *fit, double *pairs) {
+    const int i = (int)blockIdx.x;
+    const int r = rep[i];
+    if (r == i) return;
+    for (int s = (int)threadIdx.x; s < T; s += (int)blockDim.x) pairs[(size_t)i * T + s] = pairs[(size_t)r * T + s];
+    if (threadIdx.x == 0) fit[i] = fit[r];
+}
+
 // Generation counter of a captured loop and the per-generation copies that
 // read it (a graph replays with fixed arguments).
 __global__ void rows_kernel(const double *all, const int *gen, long long per_gen, double *dst) {
@@ -682,6 +731,9 @@ struct sigp_gp {
     TreeSet pset{};                   // the parents: tables + compiled arrays
     TreeSet oset{};                   // screened steps: the offspring's compiled arrays
     int32_t *order = nullptr;
+    uint64_t *thash = nullptr;        // repeated trees: token hashes,
+    int32_t *rep = nullptr;           // the first tree with the same tokens,
+    float *dcost = nullptr;           // and the launch-order cost (-1: a repeat)
     double *fit = nullptr, *pairs = nullptr, *scen = nullptr;
     int *gen = nullptr;
     int32_t *ranks = nullptr, *sel = nullptr;   // screened steps: phenotypes, next parents
@@ -713,7 +765,21 @@ static Parents parents_of(const sigp_gp *g) {
 // Analysis, compilation and launch order of the parents.
 static void launch_rebuild(sigp_gp *g) {
     analyze_kernel<<<g->P, 64, 0, g->stream>>>(g->prm, g->pset, g->mask);
-    order_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->pset.cost, g->P, g->order);
+    const unsigned nb = (unsigned)((g->P + 255) / 256);
+    tree_hash_kernel<<<nb, 256, 0, g->stream>>>(g->ptok, g->plen, g->P, g->thash);
+    tree_rep_kernel<<<nb, 256, 0, g->stream>>>(g->ptok, g->plen, g->thash, g->pset.cost, g->P, g->prm.pop, g->rep,
+                                               g->dcost);
+    order_kernel<<<nb, 256, 0, g->stream>>>(g->dcost, g->P, g->order);
+}
+
+// The parents' evaluation on the generation's rows: every distinct tree
+// simulated, then its results copied to its repeats.
+static sigp_status launch_evaluation(sigp_gp *g) {
+    const sigp_status st = sigp_batch_run(g->bt, g->scen, (int64_t)g->runs * g->T, g->pairs, g->fit, g->stream);
+    if (st != SIGP_OK) return st;
+    dup_fill_kernel<<<g->P, 64, 0, g->stream>>>(g->rep, g->T, g->fit, g->pairs);
+    GP_TRY(hipGetLastError());
+    return SIGP_OK;
 }
 
 // One step of every run: breeding, then (screened) the offspring's
@@ -745,7 +811,7 @@ static sigp_status launch_step(sigp_gp *g) {
 
 static sigp_status make_rules_batch(sigp_gp *g, const TreeSet &S, int P, const int32_t *order, int KL, int SL,
                                     bool rmod, bool rare, int T, int group_size, const sigp_launch_config &cfg,
-                                    sigp_batch **out) {
+                                    sigp_batch **out, const int32_t *rep = nullptr) {
     sigp_device_rules R;
     R.ops = S.ops;
     R.args = S.args;
@@ -754,6 +820,7 @@ static sigp_status make_rules_batch(sigp_gp *g, const TreeSet &S, int P, const i
     R.rd = S.rd;
     R.flags = S.flags;
     R.order = order;
+    R.rep = rep;
     R.P = P;
     R.KL = KL;
     R.SL = SL;
@@ -853,7 +920,8 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
                  o_gen = take(4), o_oops = take(OCL * 4), o_oargs = take(OCL * 8), o_ooff = take(OC * 4),
                  o_olen2 = take(OC * 4), o_ord2 = take(OC * 8), o_ofl = take(OC * 8), o_ocost = take(OC * 4),
                  o_ranks = take(OC * SIGP_MLP_IN * 4), o_pred = take(OC * 4), o_sel = take(screened ? P * 4 : 0),
-                 o_hash = take(OC * 8), o_dup = take(OC);
+                 o_hash = take(OC * 8), o_dup = take(OC), o_thash = take(P * 8), o_rep = take(P * 4),
+                 o_dcost = take(P * 4);
     if (hipMalloc((void **)&g->dbuf, off) != hipSuccess) {
         delete g;
         return sigp_internal_fail(SIGP_OOM, "GP allocation failed");
@@ -891,6 +959,9 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     Q.flags = (uint64_t *)(b + o_ofl);
     Q.cost = (float *)(b + o_ocost);
     g->order = (int32_t *)(b + o_ord);
+    g->thash = (uint64_t *)(b + o_thash);
+    g->rep = (int32_t *)(b + o_rep);
+    g->dcost = (float *)(b + o_dcost);
     g->fit = (double *)(b + o_fit);
     g->pairs = (double *)(b + o_pairs);
     g->scen = (double *)(b + o_scen);
@@ -923,7 +994,7 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
         lc.waves_per_item = 0;
     }
     if (status == SIGP_OK)
-        status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, lc, &g->bt);
+        status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, lc, &g->bt, g->rep);
     if (status == SIGP_OK && screened)
         status = make_rules_batch(g, Q, (int)O, nullptr, KL, SL, rmod, rare, 1, (int)O, c->launch, &g->obt);
     if (status != SIGP_OK) {
@@ -1013,7 +1084,7 @@ extern "C" sigp_status sigp_gp_evaluate(sigp_gp *g, const double *sampled, int64
     if (n_rows != (int64_t)g->runs * g->T)
         return gp_fail("need n_runs x T = " + std::to_string((long long)g->runs * g->T) + " scenario rows");
     GP_TRY(hipMemcpyAsync(g->scen, sampled, (size_t)n_rows * g->n * 8, hipMemcpyHostToDevice, g->stream));
-    sigp_status st = sigp_batch_run(g->bt, g->scen, n_rows, g->pairs, g->fit, g->stream);
+    sigp_status st = launch_evaluation(g);
     if (st != SIGP_OK) return st;
     if (fitness) GP_TRY(hipMemcpyAsync(fitness, g->fit, (size_t)g->P * 8, hipMemcpyDeviceToHost, g->stream));
     GP_TRY(hipStreamSynchronize(g->stream));
@@ -1042,7 +1113,7 @@ static sigp_status launch_generation(sigp_gp *g, const double *all, double *hist
     else if ((st = sigp_internal_sample(g->inst, &g->smp, g->run_seed, 0, g->T, g->gen_stride, g->gen_offset, g->gen,
                                         g->runs * g->T, g->scen, g->stream)) != SIGP_OK)
         return st;
-    st = sigp_batch_run(g->bt, g->scen, (int64_t)g->runs * g->T, g->pairs, g->fit, g->stream);
+    st = launch_evaluation(g);
     if (st != SIGP_OK) return st;
     record_kernel<<<(g->P + 255) / 256, 256, 0, g->stream>>>(g->fit, g->gen, g->P, hist);
     best_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->fit, g->ptok, g->plen, g->gen, best_tok, best_len);

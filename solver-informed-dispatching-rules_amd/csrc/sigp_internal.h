@@ -17,6 +17,10 @@ struct sigp_device_rules {
     const int32_t *ops = nullptr, *args = nullptr, *off = nullptr, *len = nullptr;
     const uint64_t *rd = nullptr, *flags = nullptr;
     const int32_t *order = nullptr;   // rules longest first; null = index order
+    // rep[p] != p: rule p repeats rule rep[p] (same tokens, same scenario
+    // group) and is not simulated; the caller copies rep[p]'s results.  Null:
+    // every rule is simulated.
+    const int32_t *rep = nullptr;
     int P = 0, KL = 1, SL = 1;
     bool has_rmod = false, has_rare = false;
 };

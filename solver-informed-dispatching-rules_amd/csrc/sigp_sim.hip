@@ -246,6 +246,7 @@ struct ScenSpec {
     int32_t *first;
     int first_only;          // candidate lanes: only record each pair's first decision in `first`
     long long n_ranks;       // scenario lanes: pairs of the launch (the grid is padded, see sim_kernel)
+    const int32_t *rep;      // null, or rep[p] != p: rule p repeats rule rep[p] and is skipped
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin by index.  Scenario
@@ -560,6 +561,7 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     const int pair = R.order ? R.order[rank] : rank;
     const int k = pair / R.P;
     const int p = pair - k * R.P;
+    if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
     const DevInst I = slots[k].dev;
     const FusedTabs F = slots[k].tabs;
     const int W = (int)(blockDim.x >> 6);
@@ -680,6 +682,7 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
     const int pair = R.order ? R.order[rank] : rank;
     const int k = pair / R.P;
     const int p = pair - k * R.P;
+    if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
     const DevInst I = slots[k].dev;
     const FusedTabs F = slots[k].tabs;
     const int W = (int)(blockDim.x >> 6);
@@ -792,6 +795,7 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
     const int pair = R.order ? R.order[rank] : rank;
     const int k = pair / R.P;
     const int p = pair - k * R.P;
+    if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
     const DevInst I = slots[k].dev;
     const FusedTabs F = slots[k].tabs;
     const int lane = threadIdx.x & 63;
@@ -995,6 +999,7 @@ struct sigp_batch {
     int32_t *first = nullptr;
     int32_t *perm = nullptr;
     bool order_on = false;
+    const int32_t *rep = nullptr;   // device-rule batches: repeated rules (sigp_device_rules::rep)
     int G = 1;                      // scenario groups per instance
 };
 
@@ -1692,6 +1697,7 @@ static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const 
     S.group_size = bt->group_size;
     S.nblk = (bt->T + 63) / 64;
     int mapping = resolved_mapping(bt, u);
+    S.rep = bt->rep;
     if (bt->order_on && (mapping == SIGP_MAP_SCENARIOS || mapping == SIGP_MAP_CANDIDATES_2)) {
         S.perm = bt->perm;
         S.first = bt->first;
@@ -2098,6 +2104,8 @@ sigp_status sigp_internal_batch_device(sigp_instance *inst, const sigp_device_ru
     slot.tabs = inst->tabs;
     slot.samp_off = 0;
     slot.out_off = 0;
+    slot.perm_off = 0;
+    bt->rep = r.rep;
     if (hipMalloc(&bt->dbuf, sizeof(InstSlot)) != hipSuccess) {
         delete bt;
         return fail(SIGP_OOM, "batch allocation failed");
