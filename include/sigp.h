@@ -344,7 +344,9 @@ sigp_status sigp_gp_destroy(sigp_gp *gp);
 sigp_status sigp_gp_init(sigp_gp *gp);
 /* Fitness of the population (engine/gp_engine.cu:70-79): run r's rules read
  * host scenario rows r*T .. r*T+T-1 (n_rows = n_runs*T rows of n durations);
- * fitness (nullable): n_runs*population means. */
+ * fitness (nullable): n_runs*population means.  Trees of a run with the same
+ * tokens are simulated once (their results are the same bit for bit), here
+ * and in sigp_gp_run. */
 sigp_status sigp_gp_evaluate(sigp_gp *gp, const double *sampled, int64_t n_rows, double *fitness);
 /* One gp_step of every run (engine/gp.hpp:170-194) with the given host
  * fitness (n_runs*population), or NULL for the last sigp_gp_evaluate. */
