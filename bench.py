@@ -197,7 +197,9 @@ def bench_gp_loop(args):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "GP loop on the device: clustered instance n=100 m=3 (windows U[30,120]), "
                                "%d run(s) x %d rules (NS-11 + EXTRA-18, depth <= 8), %d Gamma(4) training "
-                               "scenarios per generation, %d generations, %s"
+                               "scenarios per generation, %d generations, %s; value counts every tree of "
+                               "each generation, of which each distinct tree is simulated once (repeats "
+                               "copy its rewards, bit for bit)"
                                % (runs, pop, T, gens, "one HIP graph per generation" if not args.no_graph
                                   else "stream launches"),
                    "rules": pop * runs, "scenarios": T, "generations": gens, "n": inst.n, "m": inst.m,
