@@ -175,9 +175,10 @@ typedef struct {
  * keep their routes together longer (so do the two simulations of a
  * SIGP_MAP_CANDIDATES_2 wavefront).  Enabled by default where the batch
  * qualifies (64 <= T <= 4096, orders within 256 MiB): each pair's first
- * customer is found at creation and sigp_batch_run sorts the rows on its
- * stream before the simulation.  enable = 0 turns it off; *active (nullable)
- * reports whether it is on. */
+ * customer is found at creation (one kernel on the default stream; the
+ * creating call then waits for the device, as its uploads already do) and
+ * sigp_batch_run sorts the rows on its stream before the simulation.
+ * enable = 0 turns it off; *active (nullable) reports whether it is on. */
 sigp_status sigp_batch_order_scenarios(sigp_batch *batch, int32_t enable, int32_t *active);
 
 /* Mapping of one batch. */
