@@ -42,3 +42,25 @@ def test_library_is_newer_than_its_sources():
     srcs.append(HEADER)
     newest = max(os.path.getmtime(f) for f in srcs)
     assert os.path.getmtime(lib_path) >= newest, "libsigp.so is older than its sources: rebuild"
+
+
+def test_null_handles_are_rejected_with_a_message():
+    """Entry points that take a handle check it before any device call, so
+    these run without a GPU: each returns SIGP_INVALID_ARG and sets the
+    error text."""
+    from sigp._lib import LaunchConfig, lib
+    L = lib()
+    invalid = 1
+    cfg = LaunchConfig(0, 0)
+    on = ctypes.c_int32(0)
+    calls = [
+        ("sigp_batch_order_scenarios", lambda: L.sigp_batch_order_scenarios(None, 1, ctypes.byref(on))),
+        ("sigp_batch_configure", lambda: L.sigp_batch_configure(None, ctypes.byref(cfg))),
+        ("sigp_instance_configure", lambda: L.sigp_instance_configure(None, ctypes.byref(cfg))),
+        ("sigp_batch_run", lambda: L.sigp_batch_run(None, None, 0, None, None, None)),
+        ("sigp_batch_autotune", lambda: L.sigp_batch_autotune(None, None, 0, None, None)),
+        ("sigp_gp_step", lambda: L.sigp_gp_step(None, None)),
+    ]
+    for name, call in calls:
+        assert call() == invalid, name
+        assert L.sigp_last_error(), name
