@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <unordered_map>
 #include <cstdlib>
 #include <cmath>
 #include <cstdio>
@@ -958,6 +959,7 @@ struct sigp_instance {
     size_t mean_cap = 0;
     int waves_per_item = 0;         // default launch mapping of new batches (0 = per batch)
     int mapping = SIGP_MAP_AUTO;
+    bool skip_repeats = false;      // new batches simulate each distinct rule once
 };
 
 // Launch order and tuned mapping of the instances of one size class (one
@@ -999,7 +1001,8 @@ struct sigp_batch {
     int32_t *first = nullptr;
     int32_t *perm = nullptr;
     bool order_on = false;
-    const int32_t *rep = nullptr;   // device-rule batches: repeated rules (sigp_device_rules::rep)
+    const int32_t *rep = nullptr;   // repeated rules (sigp_device_rules::rep, or rep_owned)
+    int32_t *rep_owned = nullptr;   // host-detected repeats (sigp_instance_skip_repeats); filled by the batch
     int G = 1;                      // scenario groups per instance
 };
 
@@ -1342,6 +1345,66 @@ static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, 
 static sigp_status setup_order(sigp_batch *bt, long long perms);
 static constexpr long long ORDER_MAX_BYTES = 256LL << 20;
 
+// Repeated rules: rule p repeats rule q < p when both read the same scenario
+// group and their prepared forms (read set, scan flags, compiled code with
+// the mask applied) are equal -- then they simulate to the same rewards, so p
+// is skipped and its results copied from q after the launch.  Equality is
+// checked in full; the hash only finds candidates.
+static sigp_status find_repeats(sigp_batch *bt, const PreparedRules &pr, int group_size) {
+    const int P = pr.P;
+    std::vector<int32_t> rep(P);
+    std::unordered_map<uint64_t, std::vector<int32_t>> seen;
+    bool any = false;
+    auto same = [&](int a, int b) {
+        if (a / group_size != b / group_size || pr.rd[a] != pr.rd[b] || pr.flags[a] != pr.flags[b] ||
+            pr.lens[a] != pr.lens[b])
+            return false;
+        const int L = pr.lens[a], oa = pr.offs[a], ob = pr.offs[b];
+        for (int k = 0; k < L; ++k)
+            if (pr.ops[oa + k] != pr.ops[ob + k] || pr.args[2 * (oa + k)] != pr.args[2 * (ob + k)] ||
+                pr.args[2 * (oa + k) + 1] != pr.args[2 * (ob + k) + 1])
+                return false;
+        return true;
+    };
+    for (int p = 0; p < P; ++p) {
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](uint64_t v) { h = (h ^ v) * 1099511628211ull; };
+        mix((uint64_t)(p / group_size));
+        mix(pr.rd[p]);
+        mix(pr.flags[p]);
+        for (int k = 0; k < pr.lens[p]; ++k) {
+            const int o = pr.offs[p] + k;
+            mix((uint64_t)(uint32_t)pr.ops[o]);
+            mix((uint64_t)(uint32_t)pr.args[2 * o]);
+            mix((uint64_t)(uint32_t)pr.args[2 * o + 1]);
+        }
+        rep[p] = p;
+        std::vector<int32_t> &cands = seen[h];
+        for (int32_t q : cands)
+            if (same(p, q)) {
+                rep[p] = q;
+                any = true;
+                break;
+            }
+        if (rep[p] == p) cands.push_back(p);
+    }
+    if (!any) return SIGP_OK;
+    HIP_TRY(hipMalloc((void **)&bt->rep_owned, (size_t)P * 4));
+    HIP_TRY(hipMemcpy(bt->rep_owned, rep.data(), (size_t)P * 4, hipMemcpyHostToDevice));
+    bt->rep = bt->rep_owned;
+    return SIGP_OK;
+}
+
+// After a launch: each skipped pair (k, p) takes the rewards of (k, rep[p]).
+__global__ __launch_bounds__(256) void fill_repeats_kernel(const int32_t *__restrict__ rep, int P, int T,
+                                                           double *__restrict__ out) {
+    const int p = (int)blockIdx.x, k = (int)blockIdx.y;
+    const int r = rep[p];
+    if (r == p) return;
+    double *base = out + (size_t)k * P * T;
+    for (int s = (int)threadIdx.x; s < T; s += (int)blockDim.x) base[(size_t)p * T + s] = base[(size_t)r * T + s];
+}
+
 static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool multi,
                               const PreparedRules &pr, int T, int group_size, sigp_batch **out,
                               bool order = true) {
@@ -1475,6 +1538,13 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
         bt->units[u].tune_order = (cip)(base + o_smp[u]);
         bt->units[u].check_order = (cip)(base + o_chk[u]);
     }
+    if (insts[0]->skip_repeats && P > 1) {
+        const sigp_status st = find_repeats(bt, pr, group_size);
+        if (st != SIGP_OK) {
+            sigp_batch_destroy(bt);
+            return st;
+        }
+    }
     if (order && P > 0 && T >= 64 && T <= ORDER_MAX_T && perms * 4 <= ORDER_MAX_BYTES) {
         const sigp_status st = setup_order(bt, perms);
         if (st != SIGP_OK) {
@@ -1527,6 +1597,7 @@ extern "C" sigp_status sigp_batch_destroy(sigp_batch *bt) {
     if (bt->pairs_tmp) (void)hipFree(bt->pairs_tmp);
     if (bt->first) (void)hipFree(bt->first);
     if (bt->perm) (void)hipFree(bt->perm);
+    if (bt->rep_owned) (void)hipFree(bt->rep_owned);
     delete bt;
     return SIGP_OK;
 }
@@ -1537,6 +1608,12 @@ static sigp_status check_config(const sigp_launch_config *cfg) {
         return fail(SIGP_INVALID_ARG, "waves_per_item must be 0, 1, 2 or 4");
     if (cfg->mapping < SIGP_MAP_AUTO || cfg->mapping > SIGP_MAP_CANDIDATES_2)
         return fail(SIGP_INVALID_ARG, "mapping must be SIGP_MAP_AUTO, _SCENARIOS, _CANDIDATES or _CANDIDATES_2");
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_instance_skip_repeats(sigp_instance *inst, int32_t enable) {
+    if (!inst) return fail(SIGP_INVALID_ARG, "instance is NULL");
+    inst->skip_repeats = enable != 0;
     return SIGP_OK;
 }
 
@@ -1763,6 +1840,9 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
         }
         for (int u = 1; u < bt->n_units; ++u) HIP_TRY(hipStreamWaitEvent(s, bt->join[u - 1], 0));
     }
+    if (bt->rep_owned)
+        fill_repeats_kernel<<<dim3((unsigned)bt->P, (unsigned)bt->insts.size()), 256, 0, s>>>(bt->rep_owned, bt->P,
+                                                                                            bt->T, pairs);
     if (out_mean_dev) {
         const int rows = (int)(bt->n_out / bt->T);   // instances x rules
         mean_kernel<<<(rows + 255) / 256, 256, 0, s>>>(pairs, rows, bt->T, out_mean_dev);

@@ -156,6 +156,12 @@ class FactoryContext:
         self._h = C.c_void_p()
         check(lib().sigp_instance_create(C.byref(d), C.byref(self._h)))
 
+    def skip_repeats(self, enable: bool) -> None:
+        """Later batches and evaluate* calls simulate each distinct rule of a
+        scenario group once and copy its rewards to the repeats
+        (sigp_instance_skip_repeats)."""
+        check(lib().sigp_instance_skip_repeats(self._h, 1 if enable else 0))
+
     def configure(self, waves_per_item: int, mapping: int = MAP_AUTO) -> None:
         """Default launch mapping of later batches and evaluate* calls."""
         cfg = LaunchConfig(waves_per_item, mapping)

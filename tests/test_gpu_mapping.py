@@ -176,3 +176,43 @@ def test_scenario_order_changes_nothing(oracle):
     flat = np.concatenate([B.ravel() for B in blocks])   # instance blocks back to back
     assert_bitwise(run(bm, flat, len(insts) * 24 * T, len(insts) * T), want, "multi, ordered")
     ctx.configure(0, 0)
+
+
+def test_skip_repeats_changes_nothing(oracle):
+    """sigp_instance_skip_repeats: rule sets with repeats (within a scenario
+    group and across groups, under two mask rows, on one and on several
+    instances) give the oracle's rewards bit for bit, skipped or not."""
+    from sigp.api import FactoryContext, evaluate_multi
+    inst = load_instance("rc102")
+    base = random_rules(12, NS_TERMINALS + EXTRA, seed=41, min_depth=1, max_depth=4)
+    trees = base + base[:5] + base[2:4] + base[:1]      # 20 rules, 8 repeats
+    pk = pack_trees(trees)
+    mask = term_mask_from(NS_TERMINALS + EXTRA)
+    masks = np.stack([mask, term_mask_from(NS_TERMINALS)])
+    ctx = FactoryContext(inst)
+    T = 70
+    S = gen_durations(inst.durations, T, 9090, mode="gamma")
+    want = oracle.evaluate(inst, pk, S, T, mask, ctx.zp, threads=THREADS)
+    G, gs = 2, 10                                        # groups of 10 rules: repeats on both sides of the cut
+    SG = np.concatenate([gen_durations(inst.durations, T, 100 + g) for g in range(G)])
+    want_g = oracle.evaluate(inst, pk, SG, T, masks, ctx.zp, group_size=gs, mask_group=10, threads=THREADS)
+    insts = [inst, synthetic_mixed(n=150, m=3, seed=8)]
+    ctxs = [ctx, FactoryContext(insts[1])]
+    blocks = [gen_durations(i.durations, 24, 55 + k) for k, i in enumerate(insts)]
+    want_m = np.concatenate([oracle.evaluate(i, pk, B, 24, mask, c.zp, threads=THREADS)
+                             for i, c, B in zip(insts, ctxs, blocks)])
+    for skip in (True, False):
+        for c in ctxs:
+            c.skip_repeats(skip)
+        for m, w in [(0, 0), (1, 4), (2, 1)]:
+            ctx.configure(w, m)
+            print("skip repeats", skip, "mapping", m, flush=True)
+            assert_bitwise(ctx.evaluate(pk, S, T, mask), want, "skip %s mapping %d" % (skip, m))
+            assert_bitwise(ctx.evaluate_grouped_masks(pk, SG, T, gs, 10, masks), want_g,
+                           "grouped, skip %s mapping %d" % (skip, m))
+        ctx.configure(0, 0)
+        assert_bitwise(evaluate_multi(ctxs, pk, blocks, 24, mask), want_m, "multi, skip %s" % skip)
+        means = ctx.evaluate_means(pk, S, T, mask)[1]
+        assert np.array_equal(means, want.reshape(len(trees), T).mean(axis=1)) or \
+            np.all(means == np.array([want[p * T:(p + 1) * T].sum() / T for p in range(len(trees))]))
+    ctx.skip_repeats(False)
