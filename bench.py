@@ -213,6 +213,81 @@ def bench_gp_loop(args):
     print(json.dumps(out), flush=True)
 
 
+# The reference's published training times (results/timing/timing_bench.log:1-6,
+# its own GPU, unnamed): `timing <instance> [--terms ...] --seeds 30`.
+TIMING_PUBLISHED = {("c101", ""): 21.0, ("c101", "NSTW,MAXN,NFEAS,REACH5"): 36.0,
+                    ("r201", ""): 69.0, ("r201", "NSTW,MAXN,NFEAS,REACH5"): 134.0,
+                    ("pr15", ""): 188.0, ("pr15", "NSTW,MAXN,NFEAS,REACH5"): 1092.0}
+
+
+def bench_timing(args):
+    """s3_generation/timing.cu on the device: 30 seeds of GP (population
+    1,024, 51 generations, one training scenario per generation, NS-11 plus
+    --terms) trained side by side with the loop on the device, then each
+    seed's final rule (first maximum on one more training draw) scored on
+    500 held-out scenarios.  The timed training covers what the reference's
+    does: every generation's scenario rows (drawn on the host, as there),
+    evaluation and GP step, and the final draw."""
+    import torch
+    from sigp import gp as G
+    from sigp.api import FactoryContext, device_info
+    from sigp.gp_device import DeviceGP, tokens_to_tree
+    from sigp.instance import load_instance
+    from sigp.rules import FUNCTIONS, NS_TERMINALS, pack_trees, term_mask_from
+    from sigp.scenarios import gen_durations
+
+    name, terms = args.instance, args.terms
+    extra = [t for t in terms.split(",") if t]
+    inst = load_instance(name)
+    ctx = FactoryContext(inst)
+    vocab = list(NS_TERMINALS) + extra
+    mask = term_mask_from(vocab)
+    tc, ec = G.TreeConfig(vocab, list(FUNCTIONS)), G.EvolutionConfig()
+    seeds, T, gens = list(range(30)), ec.train_samples, ec.generations
+    log("device", device_info()[0])
+
+    def train():
+        gp = DeviceGP(ctx, tc, ec, seeds=seeds, T=T, term_mask=mask)
+        gp.init()
+        rows = np.concatenate([gen_durations(inst.durations, T, s * 10000 + g) for g in range(gens) for s in seeds])
+        gp.run(rows, generations=gens)
+        fit = gp.evaluate(np.concatenate([gen_durations(inst.durations, T, s * 10000 + gens) for s in seeds]))
+        tok, ln = gp.tokens()
+        rules = [tokens_to_tree(tok[s, int(np.argmax(fit[s])), :ln[s, int(np.argmax(fit[s]))]], tc.terminals,
+                                tc.functions) for s in seeds]
+        gp.close()
+        return rules
+
+    tw = time.perf_counter()
+    train()                                   # warm-up: code objects, graph instantiation
+    torch.cuda.synchronize()
+    log("warm-up training done in %.1f s" % (time.perf_counter() - tw))
+    t0 = time.perf_counter()
+    rules = train()
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    test = gen_durations(inst.durations, 500, 1000000)
+    t1 = time.perf_counter()
+    _, held = ctx.evaluate_means(pack_trees(rules), test, 500, mask)
+    held_s = time.perf_counter() - t1
+    log("TIMING %s [%s] vocab=%d seeds=30 | train %.2fs | heldout-eval %.3fs"
+        % (name, terms or "NSGP-baseline-11", len(vocab), train_s, held_s))
+    ref = TIMING_PUBLISHED.get((name, terms))
+    out = {
+        "metric": "GP training time, 30 seeds x 51 generations x 1,024 rules (timing.cu)",
+        "value": round(train_s, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(train_s * 1000, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None if ref is None else round(ref / train_s, 2), "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "timing %s%s --seeds 30: the reference's training on the device (same "
+                               "populations as its host loop)" % (name, (" --terms " + terms) if terms else ""),
+                   "instance": name, "vocab": len(vocab), "parallelism": "1 GPU; 30 seeds side by side"},
+        "reference_published_s": ref, "heldout_eval_s": round(held_s, 4),
+        "heldout_mean_reward": round(float(np.mean(held)), 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_sampler(args):
     """Scenario rows on the device: the large configuration's 2,000 Gamma(2)
     rows of n = 288 (clustered + uniform, m = 4), each row its own
@@ -273,7 +348,7 @@ def main():
     ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
     ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
     ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large",
-                                           "gp-loop", "sampler"],
+                                           "gp-loop", "sampler", "timing"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
@@ -282,8 +357,12 @@ def main():
                          "rules x 500 scenarios in one multi-instance batch; large: n=288 m=4, "
                          "4,096 rules x 2,000 scenarios; gp-loop: the whole GP loop on the "
                          "device (1,024 rules x 50 scenarios x 50 generations, breeding included); "
-                         "sampler: 2,000 Gamma(2) scenario rows of n=288 sampled on the device")
+                         "sampler: 2,000 Gamma(2) scenario rows of n=288 sampled on the device; "
+                         "timing: the reference's timing tool (30 seeds of GP) on the device, "
+                         "against its published training times")
     ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
+    ap.add_argument("--instance", default="c101", help="timing: instance (c101, r201, pr15 are published)")
+    ap.add_argument("--terms", default="", help="timing: extra terminals, e.g. NSTW,MAXN,NFEAS,REACH5")
     ap.add_argument("--runs", type=int, default=1, help="gp-loop: independent runs side by side")
     ap.add_argument("--no-graph", action="store_true", help="gp-loop: stream launches, no HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -307,6 +386,8 @@ def main():
         return bench_gp_loop(args)
     if args.workload == "sampler":
         return bench_sampler(args)
+    if args.workload == "timing":
+        return bench_timing(args)
     import torch
     from sigp.api import FactoryContext, batch_multi, device_info
     from sigp.workloads import (batched_uneven, gp_generation, gp_reference_generation, large,
