@@ -288,6 +288,79 @@ def bench_timing(args):
     print(json.dumps(out), flush=True)
 
 
+# The GA's 18-terminal universe, in the order of the reference's runs
+# (results/ga40/run_ga40.sh); bit b of a genome selects UNIVERSE[b].
+UNIVERSE = ["REGRET", "NSROB", "MAXN", "REACH2", "NSTW", "FRAGCNT", "FRAGSCORE", "PBUST", "REACH1", "REACH3",
+            "REACH5", "SAT", "VFRAC", "NFEAS", "HARVEST", "BOLSAO", "ATRISK", "REACH2ROB"]
+# Logged times of the reference's tools (its GPU, unnamed): the GA's first
+# batch (results/ga40/logs/c101_search.log:2) and gen_trace with all 18
+# extra terminals (results/ga40/logs_full18/full18_queue.log).
+GA_BATCH_LOGGED = {"c101": 563.9}
+TRACE_FULL18_LOGGED = {"pr01": 84.0, "c101": 154.0}
+
+
+def _timed_line(metric, value, ref, workload, extra):
+    out = {"metric": metric, "value": round(value, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": round(value * 1000, 1), "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None if ref is None else round(ref / value, 2), "dtype": "f64", "data": "synthetic",
+           "config": {"workload": workload, "parallelism": "1 GPU; GP runs side by side"},
+           "reference_logged_s": ref, "roofline": None, "cpu_baseline": None}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def bench_ga_batch(args):
+    """The GA's first batch on the device: 13 genomes (the NS-11 base and the
+    first 12 single-terminal subsets of the 18-terminal universe) x 16 seeds,
+    each the proxy fitness of s4_selection/ga.cu:109-198 (51 generations of
+    GP from Mt19937(seed), then the final rule on the 64-scenario train
+    block), with every seed's GP loop on the device."""
+    import torch
+    from sigp.api import FactoryContext, device_info
+    from sigp.ga_device import proxy_scores
+    from sigp.instance import load_instance
+    ctx = FactoryContext(load_instance(args.instance))
+    log("device", device_info()[0])
+    proxy_scores(ctx, UNIVERSE, 0, proxy=2, pgens=2)            # warm-up
+    torch.cuda.synchronize()
+    genomes = [0] + [1 << b for b in range(12)]
+    t0 = time.perf_counter()
+    for g in genomes:
+        proxy_scores(ctx, UNIVERSE, g)
+        log("genome %#x done at %.1f s" % (g, time.perf_counter() - t0))
+    dt = time.perf_counter() - t0
+    _timed_line("GA proxy evaluation time, 13 genomes x 16 seeds x 51 generations (ga.cu batch_score)", dt,
+                GA_BATCH_LOGGED.get(args.instance),
+                "ga %s first batch on the device: 13 genomes x 16 seeds, 18-terminal universe" % args.instance,
+                {"instance": args.instance})
+
+
+def bench_trace(args):
+    """gen_trace <instance> --terms <all 18> --seeds 30 on the device
+    (sigp/trace_device.py): 30 seeds x 51 generations of GP side by side,
+    per-generation records, finals on 500 held-out scenarios."""
+    import csv
+    import torch
+    from sigp.api import FactoryContext, device_info
+    from sigp.instance import load_instance
+    from sigp.trace_device import trace
+    with open(os.path.join(ROOT, "tests", "golden", "cap2-results.csv")) as f:
+        km = {r["instance"]: float(r["baseline_nsgp_mean30"]) for r in csv.DictReader(f)}[args.instance]
+    terms = [t for t in args.terms.split(",") if t] or UNIVERSE
+    ctx = FactoryContext(load_instance(args.instance))
+    log("device", device_info()[0])
+    trace(ctx, args.instance, terms, km, n_seeds=1)             # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = trace(ctx, args.instance, terms, km)
+    dt = time.perf_counter() - t0
+    log("trace %s (%d extra terminals): %.2f s, mean gain %.2f" % (args.instance, len(terms), dt, res["mean_gain"]))
+    ref = TRACE_FULL18_LOGGED.get(args.instance) if terms == UNIVERSE else None
+    _timed_line("GP trace time, 30 seeds x 51 generations (gen_trace.cu)", dt, ref,
+                "gen_trace %s --terms %s --seeds 30 on the device" % (args.instance, ",".join(terms)),
+                {"instance": args.instance, "mean_gain": res["mean_gain"]})
+
+
 def bench_sampler(args):
     """Scenario rows on the device: the large configuration's 2,000 Gamma(2)
     rows of n = 288 (clustered + uniform, m = 4), each row its own
@@ -348,7 +421,7 @@ def main():
     ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
     ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
     ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large",
-                                           "gp-loop", "sampler", "timing"],
+                                           "gp-loop", "sampler", "timing", "ga-batch", "trace"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
@@ -359,7 +432,9 @@ def main():
                          "device (1,024 rules x 50 scenarios x 50 generations, breeding included); "
                          "sampler: 2,000 Gamma(2) scenario rows of n=288 sampled on the device; "
                          "timing: the reference's timing tool (30 seeds of GP) on the device, "
-                         "against its published training times")
+                         "against its published training times; ga-batch: the GA's first batch "
+                         "of 13 genomes x 16 seeds (--instance c101 has a logged time); trace: "
+                         "gen_trace with all 18 extra terminals (--instance pr01 or c101 logged)")
     ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
     ap.add_argument("--instance", default="c101", help="timing: instance (c101, r201, pr15 are published)")
     ap.add_argument("--terms", default="", help="timing: extra terminals, e.g. NSTW,MAXN,NFEAS,REACH5")
@@ -388,6 +463,10 @@ def main():
         return bench_sampler(args)
     if args.workload == "timing":
         return bench_timing(args)
+    if args.workload == "ga-batch":
+        return bench_ga_batch(args)
+    if args.workload == "trace":
+        return bench_trace(args)
     import torch
     from sigp.api import FactoryContext, batch_multi, device_info
     from sigp.workloads import (batched_uneven, gp_generation, gp_reference_generation, large,
