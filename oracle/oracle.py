@@ -77,6 +77,12 @@ class Oracle:
         L.orc_terminal.argtypes = [vp, i32, i32, i32, dbl, i32, vp, vp]
         L.orc_terminal.restype = dbl
         L.orc_ops_read.restype = C.c_ulonglong
+        L.orc_terms_at.argtypes = [vp, i32, i32, dbl, i32, vp, vp, vp, vp]
+        L.orc_terms_at.restype = i32
+        L.orc_terms_all.argtypes = [vp, i32, dbl, i32, vp, vp, vp, vp, vp]
+        L.orc_terms_all.restype = i32
+        L.orc_sim_one_rr.argtypes = [vp, vp, vp, i32, vp, vp, vp]
+        L.orc_sim_one_rr.restype = dbl
         self.L = L
 
     def zstats(self, inst) -> np.ndarray:
@@ -129,6 +135,37 @@ class Oracle:
         z = _f64(zp)
         return self.L.orc_terminal(C.byref(s.st), slot, cand, current, ctime, remaining, _p(vis), _p(z))
 
+    def terms_at(self, inst, cand, current, ctime, remaining, visited, mask, zp):
+        """The 61 slot values of candidate `cand` (None if it is visited or
+        infeasible)."""
+        s = _Staged(inst)
+        vis = np.ascontiguousarray(visited, dtype=np.uint8)
+        mk = np.ascontiguousarray(mask, dtype=np.int64)
+        z = _f64(zp)
+        out = np.zeros(N_TERMS)
+        hit = self.L.orc_terms_at(C.byref(s.st), cand, current, ctime, remaining, _p(vis), _p(mk),
+                                  _p(z), _p(out))
+        return out if hit else None
+
+    def terms_all(self, inst, current, ctime, remaining, visited, mask, zp):
+        """(n x 61 slot values, feasible flags) of every candidate of one
+        decision; rows of infeasible or visited candidates are zero."""
+        s = _Staged(inst)
+        vis = np.ascontiguousarray(visited, dtype=np.uint8)
+        mk = np.ascontiguousarray(mask, dtype=np.int64)
+        z = _f64(zp)
+        out = np.zeros((inst.n, N_TERMS))
+        pri = np.zeros(inst.n)
+        self.L.orc_terms_all(C.byref(s.st), current, ctime, remaining, _p(vis), _p(mk), _p(z), _p(out),
+                             _p(pri))
+        return out, np.isfinite(pri)
+
+    def sim_one_rr(self, inst, ops, args, sampled_row, mask, zp) -> float:
+        s = _Staged(inst)
+        o, a = _i32(ops), _i32(args)
+        sm, mk, z = _f64(sampled_row), np.ascontiguousarray(mask, dtype=np.int64), _f64(zp)
+        return self.L.orc_sim_one_rr(C.byref(s.st), _p(o), _p(a), len(o), _p(sm), _p(mk), _p(z))
+
     def ops(self) -> int:
         return int(self.L.orc_ops_read())
 
@@ -154,6 +191,12 @@ class Reference:
         L.ref_gen_durations.argtypes = [i32, vp, dbl, i32, i64, vp]
         L.ref_compile.argtypes = [C.c_char_p, vp, vp, i32]
         L.ref_compile.restype = i32
+        L.ref_pick_best_terms.argtypes = [i32, i32, dbl] + [vp] * 6 + [i32, dbl, i32, vp, vp, vp, vp]
+        L.ref_pick_best_terms.restype = i32
+        L.ref_scan_terminal.argtypes = [i32, i32, dbl] + [vp] * 6 + [i32, i32, dbl, vp, vp, i32, vp]
+        L.ref_scan_terminal.restype = i32
+        L.ref_sim_one_rr.argtypes = [i32, i32, dbl] + [vp] * 6 + [vp, vp, i32, vp, vp, vp]
+        L.ref_sim_one_rr.restype = dbl
         self.L = L
 
     def _inst(self, inst):
@@ -183,6 +226,38 @@ class Reference:
                             *[_p(a) for a in rr], P, _p(sm), N, _p(scen_base), _p(mk), _p(mrow),
                             _p(z), threads, _p(out))
         return out
+
+    def pick_best_terms(self, inst, current, ctime, remaining, visited, mask, zp):
+        """The reference's pick_best with a constant rule: (choice, the 61
+        slot values of the last feasible candidate it evaluated)."""
+        arrs = self._inst(inst)
+        vis = np.ascontiguousarray(visited, dtype=np.uint8)
+        mk = np.ascontiguousarray(mask, dtype=np.int64)
+        z = _f64(zp)
+        out = np.full(N_TERMS, np.nan)
+        b = self.L.ref_pick_best_terms(inst.n, inst.m, float(inst.tmax), *[_p(a) for a in arrs],
+                                       current, ctime, remaining, _p(vis), _p(mk), _p(z), _p(out))
+        return b, out
+
+    def scan_terminal(self, inst, slot, current, ctime, visited, cands):
+        """The reference's terminal function of `slot`, called directly for
+        each candidate of `cands`."""
+        arrs = self._inst(inst)
+        vis = np.ascontiguousarray(visited, dtype=np.uint8)
+        cd = np.ascontiguousarray(cands, dtype=np.int32)
+        out = np.zeros(len(cd))
+        rc = self.L.ref_scan_terminal(inst.n, inst.m, float(inst.tmax), *[_p(a) for a in arrs], slot,
+                                      current, ctime, _p(vis), _p(cd), len(cd), _p(out))
+        if rc != 0:
+            raise ValueError("slot %d has no terminal function of its own" % slot)
+        return out
+
+    def sim_one_rr(self, inst, ops, args, sampled_row, mask, zp) -> float:
+        arrs = self._inst(inst)
+        o, a = _i32(ops), _i32(args)
+        sm, mk, z = _f64(sampled_row), np.ascontiguousarray(mask, dtype=np.int64), _f64(zp)
+        return self.L.ref_sim_one_rr(inst.n, inst.m, float(inst.tmax), *[_p(x) for x in arrs], _p(o),
+                                     _p(a), len(o), _p(sm), _p(mk), _p(z))
 
     def gen_durations(self, expected, n_samples, base_seed, sigma=0.2) -> np.ndarray:
         e = _f64(expected)

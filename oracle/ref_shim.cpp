@@ -58,6 +58,115 @@ void ref_evaluate(int n, int m, double tmax, const double* sc, const double* du,
     }
 }
 
+// The reference's own pick_best (engine/sim_core.hpp:43-204) on one decision
+// state, with the one-node rule "constant 0".  Its terms buffer is rewritten
+// for every feasible candidate in index order before eval_tree reads it, so
+// after the call terms_out holds all 61 slot values of the LAST feasible
+// candidate, exactly as the reference computed them under `mask`.  Returns
+// pick_best's choice (the first feasible candidate, or -1; terms_out is then
+// left as the caller filled it).
+int ref_pick_best_terms(int n, int m, double tmax, const double* sc, const double* du,
+                        const double* op, const double* cl, const double* dist, const double* d2,
+                        int current, double ctime, int remaining, const uint8_t* visited,
+                        const int64_t* mask, const double* zp, double* terms_out) {
+    SimData sd = make_sd(n, m, tmax, sc, du, op, cl, dist, d2);
+    SimBuffers buf(sd.n);
+    bool* vis = (bool*)buf.visited.data();
+    for (int i = 0; i < n; i++) vis[i] = visited[i] != 0;
+    for (int k = 0; k < N_TERMS; k++) buf.terms[k] = terms_out[k];
+    const int32_t ops[1] = {-2};
+    const int32_t args[2] = {0, 0};
+    int best = pick_best(current, ctime, remaining, ops, args, 1, sd.scores.data(),
+                         sd.durations.data(), sd.open_t.data(), sd.close_t.data(), sd.dist.data(),
+                         sd.ld, sd.dist_to_depot.data(), sd.tmax, vis, sd.n, mask, zp,
+                         buf.terms.data(), buf.stack.data(), buf.struct_buf.data(),
+                         (bool*)buf.scratch.data());
+    for (int k = 0; k < N_TERMS; k++) terms_out[k] = buf.terms[k];
+    return best;
+}
+
+// One scan terminal of engine/sim_terms.hpp, called directly with the
+// arguments pick_best passes it (engine/sim_core.hpp:99-197), for each of
+// the n_cand candidates cand[k]: out[k].  fin is the candidate's finish
+// time at the expected duration, current_time + max(open - current_time,
+// travel) + duration (engine/sim_core.hpp:86-91,98).  Slots 17-20 take
+// struct_feats_one's four outputs.  Returns -1 for a slot that no single
+// terminal function computes.
+int ref_scan_terminal(int n, int m, double tmax, const double* sc, const double* du,
+                      const double* op, const double* cl, const double* dist, const double* d2,
+                      int slot, int current, double ct, const uint8_t* visited,
+                      const int32_t* cand, int n_cand, double* out) {
+    SimData sd = make_sd(n, m, tmax, sc, du, op, cl, dist, d2);
+    SimBuffers buf(sd.n);
+    bool* vis = (bool*)buf.visited.data();
+    bool* scr = (bool*)buf.scratch.data();
+    for (int i = 0; i < n; i++) vis[i] = visited[i] != 0;
+    const double *S = sd.scores.data(), *Du = sd.durations.data(), *O = sd.open_t.data();
+    const double *Cl = sd.close_t.data(), *Di = sd.dist.data(), *D2 = sd.dist_to_depot.data();
+    const int ld = sd.ld;
+    for (int k = 0; k < n_cand; k++) {
+        int i = cand[k];
+        double ta = Di[current * ld + i + 1];
+        double to = O[i] - ct;
+        double tsv = to > ta ? to : ta;
+        double fin = ct + (tsv + Du[i]);
+        double v;
+        switch (slot) {
+        case 10: v = t_ns(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 11: v = t_regret(i, current, ct, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 12: v = t_ns_rob(i, fin, Du[i], S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 13: v = t_maxn(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 14: v = t_reach2(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 15: v = t_nstw(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 16: v = t_rand(i); break;
+        case 17: case 18: case 19: case 20: {
+            double sb[4] = {0, 0, 0, 0};
+            struct_feats_one(i, current, ct, SIM_SIGMA, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, sb);
+            v = sb[slot - 17];
+            break;
+        }
+        case 21: v = t_reach_k(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, 1, scr); break;
+        case 22: v = t_reach_k(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, 3, scr); break;
+        case 23: v = t_reach_k(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, 5, scr); break;
+        case 30: v = t_eloss_chance(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n); break;
+        case 41: v = t_reach2_rob(i, fin, Du[i], S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 44: v = t_skiploss(i, current, ct, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 47: v = t_reach_k_rob(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, 3, scr); break;
+        case 48: v = t_reach_k_rob(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, 5, scr); break;
+        case 49: v = t_ns_a2(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 50: v = t_ns_a05(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 53: v = t_ns_rob_m(i, fin, Du[i], S, Du, O, Cl, Di, ld, D2, vis, tmax, n, 0.5); break;
+        case 54: v = t_ns_rob_m(i, fin, Du[i], S, Du, O, Cl, Di, ld, D2, vis, tmax, n, 2.0); break;
+        case 55: v = t_nnfwd(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 56: v = t_nnk3(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 57: v = t_nnchain(i, current, ct, S, Du, O, Cl, Di, ld, D2, tmax, vis, n, 3, scr); break;
+        case 59: v = t_detour(i, current, fin, ta, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        case 60: v = t_scorerank(i, fin, S, Du, O, Cl, Di, ld, D2, vis, tmax, n); break;
+        default: return -1;
+        }
+        out[k] = v;
+    }
+    return 0;
+}
+
+// The reference's round-robin rollout (engine/sim_core.hpp:241-274) for one
+// rule on one scenario row.
+double ref_sim_one_rr(int n, int m, double tmax, const double* sc, const double* du,
+                      const double* op, const double* cl, const double* dist, const double* d2,
+                      const int32_t* ops, const int32_t* args, int len, const double* sampled,
+                      const int64_t* mask, const double* zp) {
+    SimData sd = make_sd(n, m, tmax, sc, du, op, cl, dist, d2);
+    SimBuffers buf(sd.n);
+    std::vector<int> cur_v(m + 1);
+    std::vector<double> time_v(m + 1);
+    std::vector<char> active(m + 1);
+    return sim_one_rr(ops, args, len, sd.scores.data(), sd.durations.data(), sd.open_t.data(),
+                      sd.close_t.data(), sd.dist.data(), sd.ld, sd.dist_to_depot.data(), sd.tmax,
+                      sampled, sd.m, sd.n, mask, zp, (bool*)buf.visited.data(), buf.terms.data(),
+                      buf.stack.data(), buf.struct_buf.data(), (bool*)buf.scratch.data(),
+                      cur_v.data(), time_v.data(), (bool*)active.data());
+}
+
 // gen_durations (engine/sim_host.hpp:50-58) with explicit expected durations.
 void ref_gen_durations(int n, const double* expected, double sigma, int n_samples,
                        long long base_seed, double* out) {

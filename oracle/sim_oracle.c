@@ -623,12 +623,18 @@ typedef struct {
     double terms[ORC_N_TERMS];
     double stack[ORC_MAX_STACK];
     double sbuf[4];
+    int cap;            /* candidate whose slot values are copied out, -1 = none */
+    double *cap_out;
+    int cap_hit;
 } orc_ws;
 
 static void ws_init(orc_ws *w, int n) {
     w->scr = (unsigned char *)calloc((size_t)n + 1, 1);
     memset(w->terms, 0, sizeof w->terms);
     memset(w->sbuf, 0, sizeof w->sbuf);
+    w->cap = -1;
+    w->cap_out = NULL;
+    w->cap_hit = 0;
 }
 static void ws_free(orc_ws *w) { free(w->scr); }
 
@@ -783,6 +789,11 @@ static int pick_best_ws(const orc_instance *I, int cur, double ct, int rem,
         T[59] = (mk[59] && NEED(59)) ? t_detour(I, i, cur, fin, ta, vis) : 0.0;
         T[60] = (mk[60] && NEED(60)) ? t_scorerank(I, i, fin, vis) : 0.0;
 
+        if (i == w->cap || w->cap == -2) {
+            memcpy(w->cap_out + (w->cap == -2 ? (size_t)i * ORC_N_TERMS : 0), T,
+                   sizeof(double) * ORC_N_TERMS);
+            w->cap_hit++;
+        }
         double pri = orc_eval_postfix(ops, args, len, T, w->stack);
         OP(2);
         if (pri != pri) pri = -1e300;
@@ -955,6 +966,34 @@ void orc_evaluate(const orc_instance *I, const int32_t *ops, const int32_t *args
         free(vis);
         ws_free(&w);
     }
+}
+
+int orc_terms_at(const orc_instance *I, int cand, int current, double ctime, int remaining,
+                 const unsigned char *visited, const int64_t *mask, const double *zp,
+                 double *out) {
+    orc_ws w;
+    ws_init(&w, I->n);
+    w.cap = cand;
+    w.cap_out = out;
+    int32_t ops[1] = {-2};
+    int32_t args[2] = {0, 0};
+    pick_best_ws(I, current, ctime, remaining, ops, args, 1, visited, mask, zp, NULL, NULL, &w);
+    ws_free(&w);
+    return w.cap_hit;
+}
+
+int orc_terms_all(const orc_instance *I, int current, double ctime, int remaining,
+                  const unsigned char *visited, const int64_t *mask, const double *zp,
+                  double *out, double *pri_out) {
+    orc_ws w;
+    ws_init(&w, I->n);
+    w.cap = -2;
+    w.cap_out = out;
+    int32_t ops[1] = {-2};
+    int32_t args[2] = {0, 0};
+    pick_best_ws(I, current, ctime, remaining, ops, args, 1, visited, mask, zp, NULL, pri_out, &w);
+    ws_free(&w);
+    return w.cap_hit;
 }
 
 double orc_terminal(const orc_instance *I, int slot, int cand, int current, double ctime,

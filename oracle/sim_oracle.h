@@ -14,10 +14,15 @@
  *   - tests/golden/gp.txt.gz: 6 x 1,024 fitness values (incl. rmod), bitwise;
  *   - the reference's own simulator compiled from its sources
  *     (oracle/_ref, see oracle/Makefile) on random rules and instances;
+ *   - per candidate, against the same reference build (tests/
+ *     test_oracle_terminals.py): all 61 slot values of the last feasible
+ *     candidate as the reference's own pick_best leaves them in its terms
+ *     buffer, and every scan terminal of engine/sim_terms.hpp called
+ *     directly for every feasible candidate, bitwise, over about 2,500
+ *     decision states on six instances;
+ *   - sim_one_rr against the reference's sim_one_rr, bitwise;
  *   - closed forms: t_rand, slots 0-9/24-27/36-40/42/58, REACH1<=REACH3<=REACH5,
  *     MAXN<=NS, brute-force greedy paths on tiny instances.
- * Parity unpinned: sim_one_rr (no caller and no fixture in the reference);
- * it is restated for completeness only.
  */
 #ifndef SIGP_SIM_ORACLE_H
 #define SIGP_SIM_ORACLE_H
@@ -79,6 +84,20 @@ void orc_evaluate(const orc_instance *I, const int32_t *ops, const int32_t *args
                   const double *sampled, int N, const int64_t *scen_base,
                   const int64_t *masks, const int32_t *mask_row, const double *zp,
                   const unsigned char *need_rows, int n_threads, double *out);
+
+/* All 61 slot values pick_best computes for candidate `cand` under `mask`
+ * (engine/sim_core.hpp:73-198; masked slots 0.0) into out; returns 1, or 0
+ * when the candidate is visited or infeasible (out untouched). */
+int orc_terms_at(const orc_instance *I, int cand, int current, double ctime, int remaining,
+                 const unsigned char *visited, const int64_t *mask, const double *zp,
+                 double *out);
+
+/* The same for every candidate of one decision: out row i (n x 61) holds
+ * candidate i's slot values where pri_out[i] (n entries) is finite, i.e.
+ * where the candidate is unvisited and feasible; returns their number. */
+int orc_terms_all(const orc_instance *I, int current, double ctime, int remaining,
+                  const unsigned char *visited, const int64_t *mask, const double *zp,
+                  double *out, double *pri_out);
 
 /* Single terminal values at a given state (tests of closed forms). */
 double orc_terminal(const orc_instance *I, int slot, int cand, int current, double ctime,
