@@ -215,9 +215,11 @@ sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_launch_confi
 /* Number of rules and (rule, scenario) pairs in a batch. */
 sigp_status sigp_batch_shape(const sigp_batch *batch, int32_t *n_rules, int64_t *n_pairs);
 
-/* Device self-test hooks for the glibc-compatible transcendentals the
- * simulator uses (erf, erfc): out[k] = f(x[k]) computed on the device.
- * which: 0 = erf, 1 = erfc, 2 = exp, 3 = sqrt, 4 = pow(x, y[k]) (rmod). */
+/* Device self-test hook for the transcendentals the simulator uses, each
+ * bitwise equal to the host C library's (glibc 2.35, x86-64 with FMA):
+ * out[k] = f(x[k]) computed on the device.  which: 0 = erf, 1 = erfc,
+ * 2 = exp, 3 = sqrt, 4 = pow(x[k], y[k]) (the rmod operator's pow,
+ * engine/sim_core.hpp:35). */
 sigp_status sigp_device_math(int32_t which, const double *x, const double *y, int64_t count,
                              double *out);
 

@@ -29,7 +29,7 @@ def build(reference: bool = True) -> None:
     """Compile the oracle (and, where the reference sources exist, _ref: the
     reference's CPU simulator, and its GPU gate and tools linked against
     libsigp.so)."""
-    targets = ["liboracle.so", "liboracle_count.so", "libcrmath.so"] + (["ref", "gate", "tools", "gpcheck"] if reference else [])
+    targets = ["liboracle.so", "liboracle_count.so", "libcrmath.so", "libpowcheck.so"] + (["ref", "gate", "tools", "gpcheck"] if reference else [])
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 
@@ -335,3 +335,26 @@ class CrMath:
     def sincos(self, x: float):
         self.L.orc_cr_sincos(x, C.byref(self._s), C.byref(self._c))
         return self._s.value, self._c.value
+
+
+class PowCheck:
+    """The device's glibc-compatible pow compiled for the host
+    (libpowcheck.so), beside libm's pow."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "libpowcheck.so")
+        if not os.path.exists(path):
+            build(reference=False)
+        L = C.CDLL(path)
+        L.orc_pow_sweep.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
+        L.orc_pow_sweep.restype = C.c_long
+        L.orc_pow_restated.argtypes = [C.c_double, C.c_double]
+        L.orc_pow_restated.restype = C.c_double
+        self.L = L
+
+    def sweep(self, x, y):
+        """(restated, libm, number of bitwise differences)."""
+        x, y = _f64(x), _f64(y)
+        mine, ref = np.zeros_like(x), np.zeros_like(x)
+        bad = self.L.orc_pow_sweep(_p(x), _p(y), x.size, _p(mine), _p(ref))
+        return mine, ref, int(bad)

@@ -5,6 +5,9 @@ For k in [0, 128): H = RN(2^(k/128)) and T = RN((2^(k/128) - H) / H), stored
 as (bits(T), bits(H) - (k << 45)) so that adding the scaled exponent k << 45
 to the second word rebuilds 2^(k/128) with the integer part folded in.
 Computed from first principles with 80-digit decimal arithmetic.
+
+Licence of the method: optimized-routines, Copyright (c) 2018 Arm Limited,
+SPDX-License-Identifier: MIT OR Apache-2.0 WITH LLVM-exception.
 """
 import struct
 from decimal import Decimal, getcontext

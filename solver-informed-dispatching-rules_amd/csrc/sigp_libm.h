@@ -10,17 +10,26 @@
 //   sigp_erf,  Sun fdlibm s_erf.c rational approximations (5 intervals),
 //   sigp_erfc  evaluated in glibc's split even/odd polynomial order, calling
 //              sigp_exp for the exp(-x^2) factor.
-//   sigp_pow_dd  pow(x, y) in double-double arithmetic (atanh series for log,
-//              Taylor series with argument halving for exp), correctly rounded
-//              except within ~2^-95 relative of a rounding midpoint.  Used by
-//              the rmod operator only.
+//   sigp_pow_glibc  pow by the same library's method (sigp_pow.h): 128-entry
+//              log table (gen_pow_log_table.py), double-double log, exp of
+//              the pair.  Used by the rmod operator only.
 //
 // Compiled with -ffp-contract=off: every fma below is explicit.
+//
+// Licences of the methods and coefficients restated here:
+//   fdlibm (erf, erfc): Copyright (C) 1993 by Sun Microsystems, Inc. All
+//     rights reserved.  Developed at SunPro, a Sun Microsystems, Inc.
+//     business.  Permission to use, copy, modify, and distribute this
+//     software is freely granted, provided that this notice is preserved.
+//   optimized-routines (exp, pow): Copyright (c) 2018 Arm Limited,
+//     SPDX-License-Identifier: MIT OR Apache-2.0 WITH LLVM-exception.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
 #include "sigp_exp_table.h"
+#include "sigp_pow.h"
+#include "sigp_pow_log_table.h"
 
 __device__ __constant__ static const uint64_t sigp_exp_tab[256] = SIGP_EXP_TAB_INIT;
 
@@ -225,108 +234,11 @@ __device__ __forceinline__ static double sigp_erfc(double x) {
     return hx > 0 ? 1e-300 * 1e-300 : 2.0 - 1e-300;
 }
 
-// ---- double-double pow for the rmod operator -------------------------------
-struct sigp_dd { double hi, lo; };
+// ---- pow for the rmod operator ---------------------------------------------
+// glibc's pow, restated in sigp_pow.h (shared with the host build that the
+// CPU tests sweep against libm), over this file's 2^(k/128) exp table.
+__device__ __constant__ static const double sigp_pow_log_tab[384] = SIGP_POW_LOG_TAB_INIT;
 
-__host__ __device__ __forceinline__ sigp_dd sigp_two_sum(double a, double b) {
-    const double s = a + b, bb = s - a;
-    return {s, (a - (s - bb)) + (b - bb)};
-}
-__host__ __device__ __forceinline__ sigp_dd sigp_fast_two_sum(double a, double b) {
-    const double s = a + b;
-    return {s, b - (s - a)};
-}
-__host__ __device__ __forceinline__ sigp_dd sigp_dd_add(sigp_dd a, sigp_dd b) {
-    sigp_dd s = sigp_two_sum(a.hi, b.hi);
-    const sigp_dd t = sigp_two_sum(a.lo, b.lo);
-    s.lo += t.hi;
-    s = sigp_fast_two_sum(s.hi, s.lo);
-    s.lo += t.lo;
-    return sigp_fast_two_sum(s.hi, s.lo);
-}
-__host__ __device__ __forceinline__ sigp_dd sigp_dd_mul(sigp_dd a, sigp_dd b) {
-    const double p = a.hi * b.hi;
-    double e = fma(a.hi, b.hi, -p);
-    e = fma(a.hi, b.lo, e);
-    e = fma(a.lo, b.hi, e);
-    return sigp_fast_two_sum(p, e);
-}
-__host__ __device__ __forceinline__ sigp_dd sigp_dd_mul_d(sigp_dd a, double b) {
-    const double p = a.hi * b;
-    double e = fma(a.hi, b, -p);
-    e = fma(a.lo, b, e);
-    return sigp_fast_two_sum(p, e);
-}
-__host__ __device__ __forceinline__ sigp_dd sigp_dd_recip_odd(int k) {   // 1/k as a double-double
-    const double h = 1.0 / (double)k;
-    return {h, fma(-h, (double)k, 1.0) / (double)k};
-}
-
-// log(x) for finite x > 0 (normal or subnormal) as a double-double.
-__host__ __device__ __forceinline__ static sigp_dd sigp_log_dd(double x) {
-    int e = 0;
-    if (x < 0x1p-1022) { x *= 0x1p54; e = -54; }
-    uint64_t u = __builtin_bit_cast(uint64_t, x);
-    e += (int)((u >> 52) & 0x7ff) - 1023;
-    double m = __builtin_bit_cast(double, (u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);   // [1, 2)
-    if (m > 1.4142135623730951) { m *= 0.5; e += 1; }                           // [0.707, 1.414]
-    // s = (m - 1) / (m + 1); m - 1 is exact (Sterbenz), m + 1 kept as a pair.
-    const double num = m - 1.0;
-    const sigp_dd den = sigp_two_sum(m, 1.0);
-    const double q1 = num / den.hi;
-    const double rem = fma(-q1, den.hi, num) - q1 * den.lo;
-    const sigp_dd s = sigp_fast_two_sum(q1, rem / den.hi);
-    const sigp_dd u2 = sigp_dd_mul(s, s);
-    // P(u) = sum_k u^k / (2k+1): terms k > 10 in double, k <= 10 in double-double.
-    double tail = 0.0;
-    for (int k = 24; k >= 11; --k) tail = fma(tail, u2.hi, 1.0 / (double)(2 * k + 1));
-    sigp_dd p = {tail, 0.0};
-    for (int k = 10; k >= 0; --k) p = sigp_dd_add(sigp_dd_recip_odd(2 * k + 1), sigp_dd_mul(p, u2));
-    sigp_dd lg = sigp_dd_mul(p, s);
-    lg.hi *= 2.0; lg.lo *= 2.0;
-    const sigp_dd ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
-    return sigp_dd_add(sigp_dd_mul_d(ln2, (double)e), lg);
-}
-
-// exp of a double-double argument, |t| < 745.
-__host__ __device__ __forceinline__ static double sigp_exp_dd(sigp_dd t) {
-    const sigp_dd ln2 = {0x1.62e42fefa39efp-1, 0x1.abc9e3b39803fp-56};
-    const double k = rint(t.hi * 1.4426950408889634);
-    const double ln2_tail = 0x1.7b57a079a1934p-111;   // ln2 beyond the pair
-    sigp_dd r = sigp_dd_add(t, sigp_dd_mul_d(ln2, -k));
-    r = sigp_dd_add(r, {-k * ln2_tail, 0.0});
-    r.hi *= 0x1p-4; r.lo *= 0x1p-4;                   // |r| < 0.022
-    sigp_dd acc = {1.0, 0.0};
-    for (int i = 16; i >= 1; --i) {                   // Horner: 1 + r/1 (1 + r/2 (1 + ...))
-        acc = sigp_dd_mul(acc, r);
-        acc = sigp_dd_mul_d(acc, 1.0 / (double)i);
-        acc.lo += fma(-(1.0 / (double)i), (double)i, 1.0) * acc.hi;   // 1/i rounding
-        acc = sigp_dd_add({1.0, 0.0}, acc);
-    }
-    for (int i = 0; i < 4; ++i) acc = sigp_dd_mul(acc, acc);
-    const int ki = (int)k;
-    const double v = acc.hi + acc.lo;                 // one rounding to 53 bits
-    if (ilogb(v) + ki >= -1022) return ldexp(v, ki);  // normal result: exact scaling
-    // subnormal result: round the double-double once on the 2^-1074 grid
-    const double hs = ldexp(acc.hi, ki + 1074), ls = ldexp(acc.lo, ki + 1074);
-    double q = rint(hs);
-    const double rem = (hs - q) + ls;
-    if (rem > 0.5) q += 1.0;
-    else if (rem < -0.5) q -= 1.0;
-    return ldexp(q, -1074);
-}
-
-// pow(x, y) for the rmod operator: x = max(a, 0) >= 0, y = 1 + clip(b, 0, 1).
-__host__ __device__ __forceinline__ static double sigp_pow_dd(double x, double y) {
-    if (y != y) return x == 1.0 ? 1.0 : y;
-    if (x != x) return x;
-    if (y == 0.0 || x == 1.0) return 1.0;
-    if (y == 1.0) return x;
-    if (x == 0.0) return y > 0.0 ? 0.0 : HUGE_VAL;
-    if (x == HUGE_VAL) return y > 0.0 ? x : 0.0;
-    const sigp_dd l = sigp_log_dd(x);
-    const sigp_dd t = sigp_dd_mul_d(l, y);
-    if (t.hi > 709.8) return HUGE_VAL;
-    if (t.hi < -745.2) return 0.0;
-    return sigp_exp_dd(t);
+__device__ __forceinline__ static double sigp_pow_glibc(double x, double y) {
+    return sigp_pow::pow(x, y, sigp_pow_log_tab, sigp_exp_tab);
 }

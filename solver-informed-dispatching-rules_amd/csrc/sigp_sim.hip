@@ -277,7 +277,7 @@ typedef const InstSlot *__restrict__ cslot;
 // (`st`, one row per level) with its top element in a register.  The stack
 // depth is wave-uniform, so every LDS access is one conflict-free row.
 // RMOD = false compiles out the rmod operator (opcode 6) for batches with no
-// such node; its double-double pow alone costs ~35 VGPRs of occupancy.
+// such node, so that the other batches carry no pow code or registers.
 template <bool RMOD>
 __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L,
                                             const double *tv, double *st) {
@@ -308,7 +308,7 @@ __device__ __forceinline__ double eval_rule(const cip ops, const cip args, int L
                 double e = b;
                 if (e < 0.0) e = 0.0;
                 else if (e > 1.0) e = 1.0;
-                r = sigp_pow_dd(base, 1.0 + e);
+                r = sigp_pow_glibc(base, 1.0 + e);
             }
             top = r;
         }
@@ -935,7 +935,7 @@ __global__ void math_kernel(int which, const double *__restrict__ x, const doubl
     case 1: r = sigp_erfc(v); break;
     case 2: r = sigp_exp(v); break;
     case 3: r = sqrt(v); break;
-    default: r = sigp_pow_dd(v, y[k]); break;
+    default: r = sigp_pow_glibc(v, y[k]); break;
     }
     out[k] = r;
 }

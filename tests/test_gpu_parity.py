@@ -9,7 +9,7 @@ import pytest
 from gpu_util import assert_bitwise
 from sigp.instance import load_instance, synthetic_clustered, synthetic_mixed, synthetic_uniform
 from sigp.rules import (EXTRA, FUNCS_RMOD, NS_TERMINALS, SLOT_NAMES, Node, pack_trees,
-                        parse_prefix, term_mask_from)
+                        parse_prefix, term_mask_from, to_prefix)
 from sigp.scenarios import gen_durations
 from sigp.workloads import random_rules
 
@@ -71,11 +71,27 @@ def test_masked_off_slots_read_zero_and_z_partners(oracle):
 
 
 def test_rmod_rules(oracle):
+    """256 rules that each hold at least one rmod node x 256 scenarios: every
+    simulation calls pow at every decision, so a last-bit difference from
+    glibc's pow that flipped any argmax would show.  The oracle computes
+    only the slots each rule reads (equal to the full evaluation bit for
+    bit, tests/test_oracle_checks.py::test_pruned_equals_full)."""
+    from oracle import need_rows_for
+    from sigp.api import FactoryContext
     inst = load_instance("c101")
-    trees = random_rules(32, NS_TERMINALS + EXTRA, seed=9, min_depth=2, max_depth=5,
-                         functions=FUNCS_RMOD)
-    S = gen_durations(inst.durations, 16, 3)
-    _check(oracle, inst, trees, S, term_mask_from(NS_TERMINALS + EXTRA), "rmod")
+    pool = random_rules(768, NS_TERMINALS + EXTRA, seed=9, min_depth=2, max_depth=5,
+                        functions=FUNCS_RMOD)
+    trees = [t for t in pool if "rmod" in to_prefix(t)][:256]
+    assert len(trees) == 256
+    S = gen_durations(inst.durations, 256, 3)
+    mask = term_mask_from(NS_TERMINALS + EXTRA)
+    pk = pack_trees(trees)
+    print("parity rmod n=%d P=%d T=%d" % (inst.n, pk.n_rules, S.shape[0]), flush=True)
+    ctx = FactoryContext(inst)
+    got = ctx.evaluate(pk, S, S.shape[0], mask)
+    want = oracle.evaluate(inst, pk, S, S.shape[0], mask, ctx.zp, need_rows=need_rows_for(pk, mask),
+                           threads=THREADS)
+    assert_bitwise(got, want, "rmod")
 
 
 def test_large_bucket_and_clustered(oracle):
