@@ -42,18 +42,22 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def work_model(wl):
+WORK_MODELS = {"plain": "work_plain_small.json", "large": "work_large.json", "batched": "work_batched.json"}
+
+
+def work_model(kind, wl):
     """Algorithmic FP64 operations per (rule, scenario) simulation of this
-    workload: the committed count of tests/golden/work_plain_small.json (made by
-    tests/golden/make_work_model.py with the counting oracle), used only when
-    the workload's arrays match the ones it was counted on."""
-    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
-    import make_work_model as mwm
-    if not os.path.exists(mwm.OUT):
+    workload: the committed count of tests/golden/work_<workload>.json (data
+    written by tests/golden/make_work_model.py with the counting oracle),
+    used only when the workload's arrays match the ones it was counted on."""
+    from sigp.workloads import workload_digest
+    name = WORK_MODELS.get(kind)
+    fn = os.path.join(ROOT, "tests", "golden", name) if name else None
+    if fn is None or not os.path.exists(fn):
         return None
-    with open(mwm.OUT) as f:
+    with open(fn) as f:
         doc = json.load(f)
-    if doc["digest"] != mwm.workload_digest(wl):
+    if doc["digest"] != workload_digest(wl):
         log("work model does not describe this workload: roofline omitted")
         return None
     return doc
@@ -299,11 +303,13 @@ GA_BATCH_LOGGED = {"c101": 563.9}
 TRACE_FULL18_LOGGED = {"pr01": 84.0, "c101": 154.0}
 
 
-def _timed_line(metric, value, ref, workload, extra):
+def _timed_line(metric, value, ref, workload, parallelism, instance, extra):
     out = {"metric": metric, "value": round(value, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
            "ms_per_step": round(value * 1000, 1), "higher_is_better": False, "scaling": "weak",
-           "vs_baseline": None if ref is None else round(ref / value, 2), "dtype": "f64", "data": "synthetic",
-           "config": {"workload": workload, "parallelism": "1 GPU; GP runs side by side"},
+           "vs_baseline": None if ref is None else round(ref / value, 2), "dtype": "f64",
+           "data": "reference instance %s (the reference's own file), service times from the "
+                   "reference's seeded streams" % instance,
+           "config": {"workload": workload, "parallelism": parallelism},
            "reference_logged_s": ref, "roofline": None, "cpu_baseline": None}
     out.update(extra)
     print(json.dumps(out), flush=True)
@@ -332,6 +338,8 @@ def bench_ga_batch(args):
     _timed_line("GA proxy evaluation time, 13 genomes x 16 seeds x 51 generations (ga.cu batch_score)", dt,
                 GA_BATCH_LOGGED.get(args.instance),
                 "ga %s first batch on the device: 13 genomes x 16 seeds, 18-terminal universe" % args.instance,
+                "1 GPU; genomes one after another, each genome's 16 seeds' GP loops side by side (the "
+                "reference ran the 13 x 16 runs as one lockstep batch)", args.instance,
                 {"instance": args.instance})
 
 
@@ -358,6 +366,7 @@ def bench_trace(args):
     ref = TRACE_FULL18_LOGGED.get(args.instance) if terms == UNIVERSE else None
     _timed_line("GP trace time, 30 seeds x 51 generations (gen_trace.cu)", dt, ref,
                 "gen_trace %s --terms %s --seeds 30 on the device" % (args.instance, ",".join(terms)),
+                "1 GPU; the 30 seeds' GP loops side by side", args.instance,
                 {"instance": args.instance, "mean_gain": res["mean_gain"]})
 
 
@@ -437,7 +446,10 @@ def main():
                          "gen_trace with all 18 extra terminals (--instance pr01 or c101 logged)")
     ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
     ap.add_argument("--instance", default="c101", help="timing: instance (c101, r201, pr15 are published)")
-    ap.add_argument("--terms", default="", help="timing: extra terminals, e.g. NSTW,MAXN,NFEAS,REACH5")
+    ap.add_argument("--terms", default="",
+                    help="timing: extra terminals beside NS-11, e.g. NSTW,MAXN,NFEAS,REACH5 (empty: NS-11 "
+                         "alone); trace: the extra terminals of the trace (empty: all 18 of the GA's "
+                         "universe)")
     ap.add_argument("--runs", type=int, default=1, help="gp-loop: independent runs side by side")
     ap.add_argument("--no-graph", action="store_true", help="gp-loop: stream launches, no HIP graph")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -545,9 +557,11 @@ def main():
 
     roof = None
     # the work model and the reference's CPU timing exist for the headline workload
-    model = None if (args.no_roofline or args.workload != "plain") else work_model(wl)
+    model = None if args.no_roofline else work_model(args.workload, wl)
     if model is not None:
-        # The simulation kernel alone (no per-rule mean kernel), same stream.
+        # The simulation kernel(s) alone (no per-rule mean kernel), same
+        # stream; a multi-instance batch forks its size classes onto side
+        # streams and joins them back, so the events bracket all of them.
         kev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
                for _ in range(args.steps)]
         for a, b in kev:

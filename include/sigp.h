@@ -205,7 +205,10 @@ sigp_status sigp_batch_autotune(sigp_batch *batch, const double *sampled_dev, in
  * would have computed.  Populations bred by GP repeat many trees.  Off by
  * default: detection hashes every rule on the host at batch creation, which
  * costs more than it saves for callers that create a batch per call around
- * a host-side GP step (the reference's GA measured 3-10 % slower with it). */
+ * a host-side GP step (the reference's GA measured 3-10 % slower with it).
+ * A multi-instance batch skips repeats only when every one of its instances
+ * has this enabled (its launch-mapping defaults come from its first
+ * instance). */
 sigp_status sigp_instance_skip_repeats(sigp_instance *inst, int32_t enable);
 
 /* Default mapping for batches created later on this instance, including the

@@ -1405,12 +1405,22 @@ __global__ __launch_bounds__(256) void fill_repeats_kernel(const int32_t *__rest
     for (int s = (int)threadIdx.x; s < T; s += (int)blockDim.x) base[(size_t)p * T + s] = base[(size_t)r * T + s];
 }
 
-static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool multi,
-                              const PreparedRules &pr, int T, int group_size, sigp_batch **out,
-                              bool order = true) {
+// Layout of a batch's device block (rules, instance slots, launch orders)
+// and its host image, computed without touching the device.
+struct BatchPlan {
+    std::vector<InstSlot> slots;
+    std::vector<int32_t> orders[3], samples[3], checks[3];
+    size_t o_ops = 0, o_args = 0, o_off = 0, o_len = 0, o_rd = 0, o_fl = 0, o_slots = 0;
+    size_t o_ord[3] = {0, 0, 0}, o_smp[3] = {0, 0, 0}, o_chk[3] = {0, 0, 0};
+    size_t bytes = 0;
+    long long perms = 0;
+};
+
+// Fills bt's host fields (shape, size classes, variant flags) and the plan.
+static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool multi, const PreparedRules &pr,
+                              int T, int group_size, sigp_batch *bt, BatchPlan &pl) {
     const int P = pr.P, K = (int)insts.size();
     if ((long long)K * P > 0x7fffffffLL) return fail(SIGP_INVALID_ARG, "instances x rules exceeds 2^31");
-    sigp_batch *bt = new sigp_batch();
     bt->insts = insts;
     bt->multi = multi;
     bt->P = P;
@@ -1423,20 +1433,21 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
     bt->has_rmod = pr.any_rmod;
     bt->has_rare = (pr.any_rd & RARE_SLOTS) != 0;
 
-    std::vector<InstSlot> slots(K);
+    pl.slots.resize(K);
     long long samp = 0, perms = 0;
     bt->G = multi || P == 0 ? 1 : (P - 1) / group_size + 1;
     for (int k = 0; k < K; ++k) {
-        slots[k].dev = insts[k]->dev;
-        slots[k].tabs = insts[k]->tabs;
-        slots[k].samp_off = samp;
-        slots[k].out_off = (long long)k * P * T;
-        slots[k].perm_off = perms;
+        pl.slots[k].dev = insts[k]->dev;
+        pl.slots[k].tabs = insts[k]->tabs;
+        pl.slots[k].samp_off = samp;
+        pl.slots[k].out_off = (long long)k * P * T;
+        pl.slots[k].perm_off = perms;
         if (multi) samp += (long long)T * insts[k]->n;
         perms += (long long)bt->G * insts[k]->n * T;
     }
+    pl.perms = perms;
     const int classes[3] = {4, 9, 16};
-    std::vector<int32_t> orders[3], samples[3], checks[3];
+    std::vector<int32_t> *orders = pl.orders, *samples = pl.samples, *checks = pl.checks;
     // the autotuner times each mapping on about 2^19 simulations of a class,
     // and checks a close ranking on about 2^21
     const long long tune_cap = std::max(1LL, (1LL << 19) / std::max(T, 1));
@@ -1491,62 +1502,99 @@ static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool mu
     }
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += align256(bytes + 8); return o; };
-    const size_t o_ops = take(pr.ops.size() * 4), o_args = take(pr.args.size() * 4),
-                 o_off = take((size_t)P * 4), o_len = take((size_t)P * 4), o_rd = take((size_t)P * 8),
-                 o_fl = take((size_t)P * 8), o_slots = take((size_t)K * sizeof(InstSlot));
-    size_t o_ord[3], o_smp[3], o_chk[3];
+    pl.o_ops = take(pr.ops.size() * 4);
+    pl.o_args = take(pr.args.size() * 4);
+    pl.o_off = take((size_t)P * 4);
+    pl.o_len = take((size_t)P * 4);
+    pl.o_rd = take((size_t)P * 8);
+    pl.o_fl = take((size_t)P * 8);
+    pl.o_slots = take((size_t)K * sizeof(InstSlot));
     for (int u = 0; u < bt->n_units; ++u) {
-        o_ord[u] = take(orders[u].size() * 4);
-        o_smp[u] = take(samples[u].size() * 4);
-        o_chk[u] = take(checks[u].size() * 4);
+        pl.o_ord[u] = take(orders[u].size() * 4);
+        pl.o_smp[u] = take(samples[u].size() * 4);
+        pl.o_chk[u] = take(checks[u].size() * 4);
     }
+    pl.bytes = off;
+    return SIGP_OK;
+}
+
+// The device block's contents, laid out as planned, in host memory.
+static std::vector<char> batch_image(const sigp_batch *bt, const BatchPlan &pl, const PreparedRules &pr) {
+    std::vector<char> img(pl.bytes, 0);
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(img.data() + o, src, bytes);
+    };
+    const int P = pr.P;
+    put(pl.o_ops, pr.ops.data(), pr.ops.size() * 4);
+    put(pl.o_args, pr.args.data(), pr.args.size() * 4);
+    put(pl.o_off, pr.offs.data(), (size_t)P * 4);
+    put(pl.o_len, pr.lens.data(), (size_t)P * 4);
+    put(pl.o_rd, pr.rd.data(), (size_t)P * 8);
+    put(pl.o_fl, pr.flags.data(), (size_t)P * 8);
+    put(pl.o_slots, pl.slots.data(), pl.slots.size() * sizeof(InstSlot));
+    for (int u = 0; u < bt->n_units; ++u) {
+        put(pl.o_ord[u], pl.orders[u].data(), pl.orders[u].size() * 4);
+        put(pl.o_smp[u], pl.samples[u].data(), pl.samples[u].size() * 4);
+        put(pl.o_chk[u], pl.checks[u].data(), pl.checks[u].size() * 4);
+    }
+    return img;
+}
+
+// Points bt's rule set, slots and launch orders into a device block `base`
+// holding batch_image's contents.
+static void bind_batch(sigp_batch *bt, char *base, const BatchPlan &pl, const PreparedRules &pr) {
+    bt->rs.ops = (cip)(base + pl.o_ops);
+    bt->rs.args = (cip)(base + pl.o_args);
+    bt->rs.off = (cip)(base + pl.o_off);
+    bt->rs.len = (cip)(base + pl.o_len);
+    bt->rs.rd = (cup)(base + pl.o_rd);
+    bt->rs.flags = (cup)(base + pl.o_fl);
+    bt->rs.order = nullptr;
+    bt->rs.P = pr.P;
+    bt->rs.KL = pr.max_k;
+    bt->rs.SL = std::max(1, pr.max_depth - 1);
+    bt->slots = (cslot)(base + pl.o_slots);
+    for (int u = 0; u < bt->n_units; ++u) {
+        bt->units[u].order = (cip)(base + pl.o_ord[u]);
+        bt->units[u].tune_order = (cip)(base + pl.o_smp[u]);
+        bt->units[u].check_order = (cip)(base + pl.o_chk[u]);
+    }
+}
+
+static sigp_status make_batch(const std::vector<sigp_instance *> &insts, bool multi,
+                              const PreparedRules &pr, int T, int group_size, sigp_batch **out,
+                              bool order = true) {
+    sigp_batch *bt = new sigp_batch();
+    BatchPlan pl;
+    sigp_status st = plan_batch(insts, multi, pr, T, group_size, bt, pl);
+    if (st != SIGP_OK) {
+        delete bt;
+        return st;
+    }
+    const std::vector<char> img = batch_image(bt, pl, pr);
     char *base = nullptr;
-    if (hipMalloc((void **)&base, off) != hipSuccess) {
+    if (hipMalloc((void **)&base, pl.bytes) != hipSuccess) {
         delete bt;
         return fail(SIGP_OOM, "batch allocation failed");
     }
     bt->dbuf = base;
-    auto up = [&](size_t o, const void *src, size_t bytes) -> bool {
-        return !bytes || hipMemcpy(base + o, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
-    };
-    bool ok = up(o_ops, pr.ops.data(), pr.ops.size() * 4) && up(o_args, pr.args.data(), pr.args.size() * 4) &&
-              up(o_off, pr.offs.data(), (size_t)P * 4) && up(o_len, pr.lens.data(), (size_t)P * 4) &&
-              up(o_rd, pr.rd.data(), (size_t)P * 8) && up(o_fl, pr.flags.data(), (size_t)P * 8) &&
-              up(o_slots, slots.data(), (size_t)K * sizeof(InstSlot));
-    for (int u = 0; ok && u < bt->n_units; ++u)
-        ok = up(o_ord[u], orders[u].data(), orders[u].size() * 4) &&
-             up(o_smp[u], samples[u].data(), samples[u].size() * 4) &&
-             up(o_chk[u], checks[u].data(), checks[u].size() * 4);
-    if (!ok) {
+    if (hipMemcpy(base, img.data(), pl.bytes, hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(base);
         delete bt;
         return fail(SIGP_DEVICE_ERROR, "batch upload failed");
     }
-    bt->rs.ops = (cip)(base + o_ops);
-    bt->rs.args = (cip)(base + o_args);
-    bt->rs.off = (cip)(base + o_off);
-    bt->rs.len = (cip)(base + o_len);
-    bt->rs.rd = (cup)(base + o_rd);
-    bt->rs.flags = (cup)(base + o_fl);
-    bt->rs.order = nullptr;
-    bt->rs.P = P;
-    bt->rs.KL = pr.max_k;
-    bt->rs.SL = std::max(1, pr.max_depth - 1);
-    bt->slots = (cslot)(base + o_slots);
-    for (int u = 0; u < bt->n_units; ++u) {
-        bt->units[u].order = (cip)(base + o_ord[u]);
-        bt->units[u].tune_order = (cip)(base + o_smp[u]);
-        bt->units[u].check_order = (cip)(base + o_chk[u]);
-    }
-    if (insts[0]->skip_repeats && P > 1) {
-        const sigp_status st = find_repeats(bt, pr, group_size);
+    bind_batch(bt, base, pl, pr);
+    bool skip = pr.P > 1;
+    for (const sigp_instance *inst : insts) skip = skip && inst->skip_repeats;
+    if (skip) {
+        st = find_repeats(bt, pr, group_size);
         if (st != SIGP_OK) {
             sigp_batch_destroy(bt);
             return st;
         }
     }
-    if (order && P > 0 && T >= 64 && T <= ORDER_MAX_T && perms * 4 <= ORDER_MAX_BYTES) {
-        const sigp_status st = setup_order(bt, perms);
+    if (order && pr.P > 0 && T >= 64 && T <= ORDER_MAX_T && pl.perms * 4 <= ORDER_MAX_BYTES) {
+        st = setup_order(bt, pl.perms);
         if (st != SIGP_OK) {
             sigp_batch_destroy(bt);
             return st;
@@ -1853,6 +1901,8 @@ extern "C" sigp_status sigp_batch_run(sigp_batch *bt, const double *sampled_dev,
 
 // Times every mapping of one size class on its sample of pairs (twice each:
 // the first run of a kernel loads its code object) and keeps the fastest.
+// Repeated rules (bt->rep) return at once here as in the runs being tuned,
+// so the sample's time reflects the work the batch will do.
 static sigp_status tune_unit(sigp_batch *bt, LaunchUnit &u, const double *sampled, hipStream_t s,
                              hipEvent_t e0, hipEvent_t e1) {
     LaunchUnit probe = u;

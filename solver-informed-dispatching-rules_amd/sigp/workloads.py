@@ -4,6 +4,7 @@ benchmark configuration, as pure functions of fixed seeds
 """
 from __future__ import annotations
 
+import hashlib
 import random
 from dataclasses import dataclass
 from typing import List
@@ -18,7 +19,7 @@ from .scenarios import gen_durations
 TEST_SEED = 1_000_000   # engine/gp_engine.cu:22
 
 __all__ = ["Workload", "MultiWorkload", "plain_small", "large", "gp_generation",
-           "gp_reference_generation", "batched_uneven", "random_rules", "TEST_SEED"]
+           "gp_reference_generation", "batched_uneven", "random_rules", "workload_digest", "TEST_SEED"]
 
 
 @dataclass
@@ -127,3 +128,19 @@ def ext29_mask() -> np.ndarray:
 
 def clustered_gp_instance() -> Instance:
     return synthetic_clustered(n=100, m=3, seed=1)
+
+
+def workload_digest(wl) -> str:
+    """sha256 over a workload's arrays: instance(s), packed rules, mask and
+    scenario rows.  A committed work model (operations per simulation) is
+    used for a workload only when its digest matches."""
+    h = hashlib.sha256()
+    insts = wl.insts if isinstance(wl, MultiWorkload) else [wl.inst]
+    for inst in insts:
+        for a in (inst.scores, inst.durations, inst.open_t, inst.close_t, inst.dist):
+            h.update(np.ascontiguousarray(a).tobytes())
+    for a in (wl.packed.ops, wl.packed.args, wl.packed.offsets, wl.packed.lengths, wl.mask):
+        h.update(np.ascontiguousarray(a).tobytes())
+    for rows in (wl.blocks if isinstance(wl, MultiWorkload) else [wl.sampled]):
+        h.update(np.ascontiguousarray(rows).tobytes())
+    return h.hexdigest()

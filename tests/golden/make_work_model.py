@@ -1,22 +1,32 @@
-"""Writes work_plain_small.json: the algorithmic FP64 work per (rule,
-scenario) simulation of the plain-method benchmark workload (1 uniform
-instance n=100 m=2, 1,024 EXTRA-18 rules, 500 Gamma(4) scenarios), counted by
-the oracle's counting build (every + - * /, compare, min/max select and libm
-call on the simulation path counts as one operation).
+"""Writes the work models of the benchmark workloads: the algorithmic FP64 work
+per (rule, scenario) simulation, counted by the oracle's counting build
+(every + - * /, compare, min/max select and libm call on the simulation path
+counts as one operation), over a fixed scenario sample.
 
-Two counts per rule over a fixed scenario sample:
+  work_plain_small.json  plain method: 1 uniform instance n=100 m=2, 1,024
+                         EXTRA-18 rules, 500 Gamma(4) scenarios; every rule
+                         over 16 scenarios (0, 31, 62, ...)
+  work_large.json        n=288 m=4, 4,096 rules (<= 127 nodes), 2,000
+                         Gamma(2) scenarios; every rule over 8 scenarios
+                         (0, 250, 500, ...)
+  work_batched.json      40 uneven instances (n 48-288) x 256 rules x 500
+                         Gamma(4) scenarios; every (instance, rule) pair over
+                         4 of its instance's scenarios (0, 125, 250, 375)
+
+Two counts per rule (per (instance, rule) pair for the batch):
   ops       -- the work the evaluator has to do: only the terminal slots the
                rule reads and the mask makes live (plus their dependencies);
   ops_full  -- the work as the reference writes it (engine/sim_core.hpp:99-198
-               computes every masked-in slot for every feasible candidate).
+               computes every masked-in slot for every feasible candidate);
+               counted for the plain workload only.
 bench.py prices its roofline with the mean of `ops`.
 
-Run from the repository root:  python tests/golden/make_work_model.py
+Run from the repository root:  python tests/golden/make_work_model.py [plain|large|batched]
 """
-import hashlib
 import json
 import os
 import sys
+from multiprocessing import Pool
 
 import numpy as np
 
@@ -25,60 +35,78 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "solver-informed-dispatching-rules_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-OUT = os.path.join(HERE, "work_plain_small.json")
-SCEN_STRIDE = 31       # scenarios 0, 31, 62, ... (16 of the 500)
-N_SAMPLE_SCEN = 16
+FILES = {"plain": "work_plain_small.json", "large": "work_large.json", "batched": "work_batched.json"}
+OUT = os.path.join(HERE, FILES["plain"])
+SAMPLE = {"plain": (31, 16), "large": (250, 8), "batched": (125, 4)}   # (stride, count)
 
 
-def workload_digest(wl) -> str:
-    """sha256 over the workload's arrays (instance, packed rules, mask, scenarios)."""
-    h = hashlib.sha256()
-    for a in (wl.inst.scores, wl.inst.durations, wl.inst.open_t, wl.inst.close_t, wl.inst.dist,
-              wl.packed.ops, wl.packed.args, wl.packed.offsets, wl.packed.lengths, wl.mask,
-              wl.sampled):
-        h.update(np.ascontiguousarray(a).tobytes())
-    return h.hexdigest()
+def path(kind: str) -> str:
+    return os.path.join(HERE, FILES[kind])
 
 
-def sample_rows(wl) -> np.ndarray:
-    return np.ascontiguousarray(wl.sampled[::SCEN_STRIDE][:N_SAMPLE_SCEN])
+def workload(kind: str):
+    from sigp.workloads import batched_uneven, large, plain_small
+    return {"plain": plain_small, "large": large, "batched": batched_uneven}[kind]()
 
 
-def count_rule(O, wl, p, S, zp, pruned: bool) -> int:
+def sample_rows(wl, kind: str = "plain", inst_index: int = 0) -> np.ndarray:
+    stride, count = SAMPLE[kind]
+    rows = wl.blocks[inst_index] if kind == "batched" else wl.sampled
+    return np.ascontiguousarray(rows[::stride][:count])
+
+
+def count_rule(O, wl, p, S, zp, pruned: bool, inst=None) -> int:
     """FP64 operations of rule p over the sample rows S."""
     from oracle import need_rows_for
     from sigp.rules import pack_trees
     pk = pack_trees([wl.trees[p]])
     O.reset_ops()
-    O.evaluate(wl.inst, pk, S, S.shape[0], wl.mask, zp,
+    O.evaluate(inst if inst is not None else wl.inst, pk, S, S.shape[0], wl.mask, zp,
                need_rows=need_rows_for(pk, wl.mask) if pruned else None, threads=1)
     return O.ops()
 
 
-def main():
+_W = {}
+
+
+def _init(kind):
     from oracle import Oracle
-    from sigp.workloads import plain_small
-    wl = plain_small()
-    O = Oracle(counting=True)
-    zp = O.zstats(wl.inst)
-    S = sample_rows(wl)
-    ops = [count_rule(O, wl, p, S, zp, True) for p in range(wl.packed.n_rules)]
-    ops_full = [count_rule(O, wl, p, S, zp, False) for p in range(wl.packed.n_rules)]
-    n_sims = wl.packed.n_rules * S.shape[0]
+    _W["kind"], _W["wl"], _W["O"] = kind, workload(kind), Oracle(counting=True)
+    insts = _W["wl"].insts if kind == "batched" else [_W["wl"].inst]
+    _W["zp"] = [_W["O"].zstats(i) for i in insts]
+
+
+def _job(item):
+    k, p, full = item
+    kind, wl, O = _W["kind"], _W["wl"], _W["O"]
+    inst = wl.insts[k] if kind == "batched" else wl.inst
+    return count_rule(O, wl, p, sample_rows(wl, kind, k), _W["zp"][k], not full, inst)
+
+
+def main(kind: str = "plain") -> None:
+    from sigp.workloads import workload_digest
+    wl = workload(kind)
+    n_inst = len(wl.insts) if kind == "batched" else 1
+    P = wl.packed.n_rules
+    items = [(k, p, False) for k in range(n_inst) for p in range(P)]
+    with Pool(min(8, os.cpu_count() or 1), initializer=_init, initargs=(kind,)) as pool:
+        ops = pool.map(_job, items, chunksize=4)
+        ops_full = pool.map(_job, [(k, p, True) for k, p, _ in items], chunksize=4) if kind == "plain" else None
+    n_sims = len(items) * SAMPLE[kind][1]
     doc = {
         "workload": wl.name,
         "digest": workload_digest(wl),
-        "scenario_sample": {"stride": SCEN_STRIDE, "count": int(S.shape[0])},
+        "scenario_sample": {"stride": SAMPLE[kind][0], "count": SAMPLE[kind][1]},
         "ops_per_sim": sum(ops) / n_sims,
-        "ops_full_per_sim": sum(ops_full) / n_sims,
         "ops": ops,
-        "ops_full": ops_full,
     }
-    with open(OUT, "w") as f:
+    if ops_full is not None:
+        doc["ops_full_per_sim"] = sum(ops_full) / n_sims
+        doc["ops_full"] = ops_full
+    with open(path(kind), "w") as f:
         json.dump(doc, f)
-    print("ops/sim %.1f (as written %.1f) over %d sims -> %s"
-          % (doc["ops_per_sim"], doc["ops_full_per_sim"], n_sims, OUT))
+    print("%s: ops/sim %.1f over %d sims -> %s" % (kind, doc["ops_per_sim"], n_sims, path(kind)))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else "plain")

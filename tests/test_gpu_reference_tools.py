@@ -9,13 +9,14 @@ reference's stored results on MI355X:
 * include/sigp_gp.hpp: evolve_one_device (the GP loop on the device) returns
   the population of the reference's own generation loop (its engine/gp.hpp
   operators on the host) tree for tree;
-* s4_selection/ga.cu: the proxy fitness of the GA's initial pool (19 terminal
-  subsets x 16 seeds x 51 generations) equals the hexfloats of the stored
-  evaluation cache results/ga40/logs/c204.evcache, except the subset
-  {ATRISK}, whose stored values come from the reference's CUDA build (its
-  erf differs from glibc, verify/verify_cuda.cu:145-149; this library is
-  bitwise to the CPU simulator).  About 3.5 minutes: runs with
-  SIGP_SLOW_TESTS=1.
+* s4_selection/ga.cu: the tool's own --gate passes; the proxy fitness of a
+  reduced initial pool (base + 3 singles x 16 seeds x 51 generations)
+  equals the stored cache lines of results/ga40/logs/c204.evcache; the
+  whole initial pool (19 terminal subsets, about 3.5 minutes, with
+  SIGP_SLOW_TESTS=1) equals it except the subset {ATRISK}, whose stored
+  values come from the reference's CUDA build (its erf differs from glibc,
+  verify/verify_cuda.cu:145-149; this library is bitwise to the CPU
+  simulator).
 
 The binaries exist where the reference sources were present at build time;
 elsewhere these tests skip.  Fixtures: tests/golden/traces, gacache and
@@ -77,6 +78,45 @@ def _cache(lines):
         if p and p[0] == "P":
             P[int(p[1], 16)] = p[3:]
     return P
+
+
+def _run_ga(args, cwd, timeout):
+    """Runs the reference's ga with its progress lines echoed as they come
+    (line-buffered, s4_selection/ga.cu:52,202); returns (rc, output)."""
+    proc = subprocess.Popen([_tool("ga")] + args, cwd=cwd, env=_env(), stdout=subprocess.PIPE,
+                            stderr=subprocess.STDOUT, text=True)
+    out = []
+    for line in proc.stdout:
+        print(line.rstrip()[:200], flush=True)
+        out.append(line)
+    return proc.wait(timeout=timeout), "".join(out)
+
+
+def test_ga_gate_passes(tmp_path):
+    """The tool's own gate (s4_selection/ga.cu:267-290): its batched
+    evaluate_grouped_masks path against per-genome evaluate_grouped /
+    evaluate, 5 genomes (base, all 18, two singles, a triple) x 4 seeds x 5
+    generations, on this library; every value must match."""
+    rc, out = _run_ga(["c204", "--universe", UNIVERSE, "--gate"], tmp_path, 600)
+    assert rc == 0, out[-2000:]
+    assert "GATE_PASS" in out and "mismatches 0" in out
+
+
+def test_ga_reduced_initial_pool_reproduces_stored_cache(tmp_path):
+    """The GA's initial pool over a 3-terminal universe (REGRET, NSROB,
+    MAXN): the base genome and the three singles, each 16 seeds x 51
+    generations of GP on the library.  A genome's vocabulary is NS-11 plus
+    its terminals in universe order, so these four are the stored run's
+    genomes 0, 1, 2 and 4, whose cache lines (16 hexfloats each) must match."""
+    rc, out = _run_ga(["c204", "--universe", "REGRET,NSROB,MAXN", "--seeds", "neutral", "--nofull", "--pm", "0.5",
+                       "--gens", "0", "--stag", "10", "--top", "1", "--confirm", "1", "--bmax", "4",
+                       "--cache", "c204.evcache"], tmp_path, 600)
+    assert rc == 0, out[-2000:]
+    got = _cache(open(tmp_path / "c204.evcache"))
+    want = _cache(gzip.open(os.path.join(GOLDEN, "gacache", "c204.evcache.gz"), "rt"))
+    for g in (0, 1, 2, 4):
+        assert g in got and g in want
+        assert got[g] == want[g], hex(g)
 
 
 @pytest.mark.skipif(os.environ.get("SIGP_SLOW_TESTS") != "1", reason="about 3.5 minutes; set SIGP_SLOW_TESTS=1")
