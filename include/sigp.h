@@ -135,6 +135,29 @@ sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t n_inst, con
                                 int32_t n_masks, int32_t mask_group, double *out_pairs,
                                 double *out_mean);
 
+/* sigp_evaluate without device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as code
+ * under HIP graph capture, tight serving loops and harnesses that forbid
+ * hipMalloc.  Same arguments and results as sigp_evaluate, but the scenario
+ * rows (sampled_dev) and the results (out_pairs_dev, required; out_mean_dev,
+ * nullable) are device pointers, and the work goes on `stream` (a
+ * hipStream_t, NULL = default stream); the call returns when the results
+ * are written.  The block sigp_batch_create would allocate (compiled rules,
+ * read sets, the launch order: about 28 bytes per rule plus 12 per postfix
+ * node) lives in a fixed device array of SIGP_NOALLOC_MAX_BYTES per GPU,
+ * held until the process ends; rule sets that need more are rejected with
+ * SIGP_INVALID_ARG.  No scenario order, repeated-rule skipping or tuning
+ * (results are identical without them).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (other threads, other streams) race
+ * and must be serialised by the caller.  Host-side rule validation uses
+ * ordinary heap memory. */
+#define SIGP_NOALLOC_MAX_BYTES (8 << 20)
+sigp_status sigp_evaluate_noalloc(sigp_instance *inst, const sigp_rules *rules,
+                                  const double *sampled_dev, int64_t n_rows, int32_t T,
+                                  int32_t group_size, const int64_t *masks, int32_t n_masks,
+                                  int32_t mask_group, double *out_pairs_dev, double *out_mean_dev,
+                                  void *stream);
+
 /* Launch mapping of the simulation kernel (no counterpart in the reference,
  * whose kernels always run one thread per (rule, scenario) pair).  Every
  * mapping gives bit-identical results.

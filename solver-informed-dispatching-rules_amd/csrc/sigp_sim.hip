@@ -2095,6 +2095,87 @@ extern "C" sigp_status sigp_evaluate_multi(sigp_instance *const *insts, int32_t 
 
 // Phenotype launch on `stream`: one workgroup per rule, as many wavefronts
 // as LDS allows beside the kernel's 1024 static priorities.
+// ---- allocation-free evaluation --------------------------------------------
+// sigp_evaluate_noalloc keeps the block sigp_batch_create would hipMalloc --
+// compiled rules, read sets, fused-computation flags, the instance slot and
+// the launch order -- in this file-scope array instead: one copy per GPU,
+// held until the process ends.  Host code never names the array when it
+// needs its memory: the device address comes from hipGetSymbolAddress, once
+// per device, with that device current.
+static __device__ __align__(256) char scratch[SIGP_NOALLOC_MAX_BYTES];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+static sigp_status scratch_on_current_device(char **out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(SIGP_INVALID_ARG, "device index " + std::to_string(dev) + " beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(SIGP_DEVICE_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_evaluate_noalloc(sigp_instance *inst, const sigp_rules *rules,
+                                             const double *sampled_dev, int64_t n_rows, int32_t T,
+                                             int32_t group_size, const int64_t *masks, int32_t n_masks,
+                                             int32_t mask_group, double *out_pairs_dev, double *out_mean_dev,
+                                             void *stream) {
+    if (!inst || !rules) return fail(SIGP_INVALID_ARG, "inst and rules must be non-NULL");
+    if (T < 0) return fail(SIGP_INVALID_ARG, "T < 0");
+    if (group_size < 1) return fail(SIGP_INVALID_ARG, "group_size must be >= 1");
+    PreparedRules pr;
+    sigp_status st = prepare_rules(rules, masks, n_masks, mask_group, pr);
+    if (st != SIGP_OK) return st;
+    if (pr.P == 0 || T == 0) {
+        if (out_mean_dev && pr.P > 0) return fail(SIGP_INVALID_ARG, "mean over T = 0 scenarios");
+        return SIGP_OK;
+    }
+    if (!sampled_dev) return fail(SIGP_INVALID_ARG, "sampled_dev is NULL");
+    if (!out_pairs_dev) return fail(SIGP_INVALID_ARG, "out_pairs_dev is NULL (the rewards are the caller's buffer)");
+    sigp_batch bt;   // host-side description; its device block is the scratch array
+    BatchPlan pl;
+    st = plan_batch({inst}, false, pr, T, group_size, &bt, pl);
+    if (st != SIGP_OK) return st;
+    if (pl.bytes > SIGP_NOALLOC_MAX_BYTES)
+        return fail(SIGP_INVALID_ARG, "these rules need " + std::to_string(pl.bytes) +
+                                          " bytes of device scratch; sigp_evaluate_noalloc holds at most " +
+                                          std::to_string((long long)SIGP_NOALLOC_MAX_BYTES) +
+                                          " (SIGP_NOALLOC_MAX_BYTES): use sigp_evaluate or fewer rules");
+    if (n_rows < bt.rows_needed)
+        return fail(SIGP_INVALID_ARG, "sampled has " + std::to_string(n_rows) + " rows, rules need " +
+                                          std::to_string(bt.rows_needed));
+    char *ptr = nullptr;
+    st = scratch_on_current_device(&ptr);
+    if (st != SIGP_OK) return st;
+    const std::vector<char> img = batch_image(&bt, pl, pr);
+    hipStream_t s = (hipStream_t)stream;
+    // every byte the kernels read is rewritten here, so nothing left by the
+    // previous call matters
+    HIP_TRY(hipMemcpyAsync(ptr, img.data(), pl.bytes, hipMemcpyHostToDevice, s));
+    bind_batch(&bt, ptr, pl, pr);
+    st = launch_unit(&bt, bt.units[0], sampled_dev, out_pairs_dev, s);
+    if (st != SIGP_OK) return st;
+    if (out_mean_dev) {
+        mean_kernel<<<(pr.P + 255) / 256, 256, 0, s>>>(out_pairs_dev, pr.P, T, out_mean_dev);
+        HIP_TRY(hipGetLastError());
+    }
+    // synchronous: the host image must outlive the copy, and the scratch
+    // array is free for the next call once this one returns
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(SIGP_DEVICE_ERROR, std::string("kernel failed: ") + hipGetErrorString(e));
+    return SIGP_OK;
+}
+
 static void launch_pheno(const sigp_batch *bt, const PhenoSpec &Ph, int32_t *outd, hipStream_t stream) {
     int w = 4;
     while (w > 1 && 1024 * sizeof(double) + (size_t)w * lds_per_wave(bt) > LDS_LIMIT) w >>= 1;

@@ -19,7 +19,7 @@ lib_path = os.path.join(_HERE, "libsigp.so")
 
 # Every symbol include/sigp.h declares.
 EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp_instance_destroy",
-            "sigp_zstats", "sigp_evaluate", "sigp_batch_create", "sigp_batch_run",
+            "sigp_zstats", "sigp_evaluate", "sigp_evaluate_noalloc", "sigp_batch_create", "sigp_batch_run",
             "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math", "sigp_batch_configure",
             "sigp_batch_order_scenarios", "sigp_instance_skip_repeats",
             "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
@@ -110,6 +110,7 @@ def lib() -> C.CDLL:
         "sigp_instance_destroy": ([vp], C.c_int),
         "sigp_zstats": ([vp, vp], C.c_int),
         "sigp_evaluate": ([vp, vp, vp, i64, i32, i32, vp, i32, i32, vp, vp], C.c_int),
+        "sigp_evaluate_noalloc": ([vp, vp, vp, i64, i32, i32, vp, i32, i32, vp, vp, vp], C.c_int),
         "sigp_batch_create": ([vp, vp, i32, i32, vp, i32, i32, vp], C.c_int),
         "sigp_batch_run": ([vp, vp, i64, vp, vp, vp], C.c_int),
         "sigp_batch_destroy": ([vp], C.c_int),

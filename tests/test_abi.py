@@ -60,6 +60,8 @@ def test_null_handles_are_rejected_with_a_message():
         ("sigp_batch_run", lambda: L.sigp_batch_run(None, None, 0, None, None, None)),
         ("sigp_batch_autotune", lambda: L.sigp_batch_autotune(None, None, 0, None, None)),
         ("sigp_gp_step", lambda: L.sigp_gp_step(None, None)),
+        ("sigp_evaluate_noalloc", lambda: L.sigp_evaluate_noalloc(None, None, None, 0, 0, 1, None, 0, 1,
+                                                                  None, None, None)),
     ]
     for name, call in calls:
         assert call() == invalid, name
