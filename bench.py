@@ -343,6 +343,46 @@ def bench_ga_batch(args):
                 {"instance": args.instance})
 
 
+# The reference's logged whole GA search on c204 (results/ga40/logs/
+# c204_search.log): 85 evaluations (80 at 16 seeds, 5 confirmations at 30)
+# in 30,236 s of wall time on its unnamed GPU, resumed from 26 cached ones.
+GA_SEARCH_LOGGED = {"c204": (30236.0, 85)}
+
+
+def bench_ga_search(args):
+    """The GA terminal-subset search (s4_selection/ga.cu:267-520, restated in
+    sigp/ga_search.py) with the proxy fitness of every genome trained by the
+    device GP loop: neutral seeds, 13-genome tree, generations until
+    stagnation (10), top-5 finalists confirmed at 30 seeds.  Starts from an
+    empty cache, so every genome the search visits is trained here."""
+    import torch
+    from sigp.api import FactoryContext, device_info
+    from sigp.ga_device import proxy_batch
+    from sigp.ga_search import ga_search
+    from sigp.instance import load_instance
+    ctx = FactoryContext(load_instance(args.instance))
+    log("device", device_info()[0])
+    from sigp.ga_device import proxy_scores
+    proxy_scores(ctx, UNIVERSE, 0, proxy=2, pgens=2)            # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = ga_search(UNIVERSE, proxy_batch(ctx, UNIVERSE), proxy_batch(ctx, UNIVERSE, proxy=30),
+                    seed_mode="neutral", nofull=True, pm=0.5, gens=50, stag_k=10, top_n=5,
+                    log=lambda s: log("%7.1f s  %s" % (time.perf_counter() - t0, s)))
+    dt = time.perf_counter() - t0
+    ref = GA_SEARCH_LOGGED.get(args.instance)
+    per_eval_ref = ref[0] / ref[1] if ref else None
+    _timed_line("GA terminal-subset search wall time (ga.cu, all proxy evaluations on the device)", dt,
+                None, "ga %s --seeds neutral --nofull --pm 0.5 --gens 50 --stag 10 --top 5 on the device, "
+                "empty cache" % args.instance,
+                "1 GPU; genomes one after another, each genome's 16 (confirmation: 30) seeds' GP loops side by "
+                "side", args.instance,
+                {"instance": args.instance, "evals": res.evals, "s_per_eval": round(dt / max(1, res.evals), 3),
+                 "reference_logged_s_per_eval": None if per_eval_ref is None else round(per_eval_ref, 1),
+                 "speedup_per_eval": None if per_eval_ref is None else round(per_eval_ref / (dt / res.evals), 1),
+                 "champion": res.lines[-1]})
+
+
 def bench_trace(args):
     """gen_trace <instance> --terms <all 18> --seeds 30 on the device
     (sigp/trace_device.py): 30 seeds x 51 generations of GP side by side,
@@ -430,7 +470,7 @@ def main():
     ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
     ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
     ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large",
-                                           "gp-loop", "sampler", "timing", "ga-batch", "trace"],
+                                           "gp-loop", "sampler", "timing", "ga-batch", "ga-search", "trace"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
@@ -442,7 +482,8 @@ def main():
                          "sampler: 2,000 Gamma(2) scenario rows of n=288 sampled on the device; "
                          "timing: the reference's timing tool (30 seeds of GP) on the device, "
                          "against its published training times; ga-batch: the GA's first batch "
-                         "of 13 genomes x 16 seeds (--instance c101 has a logged time); trace: "
+                         "of 13 genomes x 16 seeds (--instance c101 has a logged time); ga-search: the "
+                         "whole GA terminal-subset search (--instance c204 has a logged time); trace: "
                          "gen_trace with all 18 extra terminals (--instance pr01 or c101 logged)")
     ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
     ap.add_argument("--instance", default="c101", help="timing: instance (c101, r201, pr15 are published)")
@@ -477,6 +518,8 @@ def main():
         return bench_timing(args)
     if args.workload == "ga-batch":
         return bench_ga_batch(args)
+    if args.workload == "ga-search":
+        return bench_ga_search(args)
     if args.workload == "trace":
         return bench_trace(args)
     import torch

@@ -59,3 +59,13 @@ def proxy_scores(ctx: FactoryContext, universe: Sequence[str], genome: int, prox
     blk = gen_durations(dur, N_TRAIN, TRAIN_SEED)
     _, means = ctx.evaluate_means(pack_trees(finals), blk, N_TRAIN, mask)
     return means
+
+
+def proxy_batch(ctx: FactoryContext, universe: Sequence[str], proxy: int = 16, pgens: int = 51):
+    """A proxy function for sigp.ga_search.ga_search: genomes -> their
+    per-seed values (the tool's batch_score, s4_selection/ga.cu:109-198, at
+    `proxy` seeds and `pgens` generations), one genome's seeds side by side
+    on the device at a time."""
+    def run(genomes):
+        return [list(proxy_scores(ctx, universe, g, proxy=proxy, pgens=pgens)) for g in genomes]
+    return run
