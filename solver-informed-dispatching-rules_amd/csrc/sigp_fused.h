@@ -227,13 +227,19 @@ __device__ __forceinline__ double scan_nstw(const FusedTabs &F, int n, int c, do
 
 // ATRISK (engine/sim_terms.hpp:225-257) for candidate c.  pow(var, 0.5)
 // comes from the per-instance duration-class table.  One neighbour at a
-// time: the body holds two erf evaluations.
+// time: the body holds two erf evaluations.  A lane whose difference lies
+// beyond its divisor's saturation threshold (DevInst::atp*, atq*) takes the
+// value norm_cdf would return there, exactly 1 or 0, without the division
+// and the erf; the erf path runs only for lanes that need it.
 template <int NW>
 __device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &F, int c, double fin_c,
                              const uint32_t (&vis)[NW], const uint32_t (&cnow)[NW]) {
     const double sdc = F.sdv[c];
     const double sdc2 = sdc * sdc;
-    const cdp pwrow = I.pw2 + (size_t)I.dcls[c] * I.K;
+    const int kc = I.dcls[c];
+    const cdp pwrow = I.pw2 + (size_t)kc * I.K;
+    const cdp qrow = I.atq2 + (size_t)kc * I.K * 2;
+    const double p_hi = I.atp1[kc], p_lo = I.atp0[kc];
     const cdp dd = F.d + (size_t)(c + 1) * F.ldp;
     double el = 0.0;
     uint32_t mb = 0;
@@ -245,7 +251,14 @@ __device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &
         if (!act) continue;
         const double ma = fin_c + dd[j];
         const double cl = F.close[j];
-        const double pw = sdc > 1e-9 ? norm_cdf((cl - ma) / sdc) : (ma <= cl ? 1.0 : 0.0);
+        double pw;
+        if (sdc > 1e-9) {
+            const double dc = cl - ma;
+            pw = dc >= p_hi ? 1.0 : 0.0;
+            if (dc < p_hi && dc > p_lo) pw = norm_cdf(dc / sdc);
+        } else {
+            pw = ma <= cl ? 1.0 : 0.0;
+        }
         double pb = 1.0;
         if (pw != 0.0) {   // 1 - 0 * pb == 1 for every pb in [0, 1]
             const double op = F.open[j];
@@ -257,7 +270,11 @@ __device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &
             const double mbd = ms + F.dur[j] + F.d2d[j];
             if (vb > 1e-18) {
                 const int kj = I.dcls[j];
-                pb = norm_cdf((I.tmax - mbd) / (late ? pwrow[kj] : I.pw1[kj]));
+                const double e = I.tmax - mbd;
+                const double q_hi = late ? qrow[2 * kj] : I.atq1[2 * kj];
+                const double q_lo = late ? qrow[2 * kj + 1] : I.atq1[2 * kj + 1];
+                pb = e >= q_hi ? 1.0 : 0.0;
+                if (e < q_hi && e > q_lo) pb = norm_cdf(e / (late ? pwrow[kj] : I.pw1[kj]));
             } else {
                 pb = mbd <= I.tmax ? 1.0 : 0.0;
             }

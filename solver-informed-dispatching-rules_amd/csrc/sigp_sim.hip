@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <unordered_map>
 #include <cstdlib>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1082,6 +1083,47 @@ static void launch_zstats(const DevInst &I, double *feat, int *ok, double *zp) {
 static int nw_bucket(int n) { return n <= 128 ? 4 : (n <= 288 ? 9 : 16); }
 
 // Builds the device tables of an instance into one allocation.
+// Order-preserving 64-bit keys of finite doubles, for bisection over them.
+static uint64_t dkey(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+static double dkey_inv(uint64_t k) {
+    const uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double x;
+    std::memcpy(&x, &u, 8);
+    return x;
+}
+// The erf argument of norm_cdf(d / s) (engine/sim_terms.hpp:221-223), with
+// its two roundings; non-decreasing in d for s > 0.
+static double ncdf_arg(double d, double s) {
+    const double z = d / s;
+    return z * 0.7071067811865476;
+}
+// Smallest d with ncdf_arg(d, s) >= 6: from there erf returns 1 - 1e-300
+// == 1 and norm_cdf exactly 1.
+static double ncdf_one_from(double s) {
+    uint64_t lo = dkey(-DBL_MAX), hi = dkey(DBL_MAX);
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (ncdf_arg(dkey_inv(mid), s) >= 6.0) hi = mid;
+        else lo = mid;
+    }
+    return dkey_inv(hi);
+}
+// Largest d with ncdf_arg(d, s) <= -6: up to there erf returns 1e-300 - 1
+// == -1 and norm_cdf exactly 0.
+static double ncdf_zero_to(double s) {
+    uint64_t lo = dkey(-DBL_MAX), hi = dkey(DBL_MAX);
+    while (hi - lo > 1) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (ncdf_arg(dkey_inv(mid), s) <= -6.0) lo = mid;
+        else hi = mid;
+    }
+    return dkey_inv(lo);
+}
+
 static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I) {
     const int n = d->n, ld = n + 1;
     I->n = n;
@@ -1123,6 +1165,26 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
         durrob[i] = d->durations[i] * (1.0 + kSigma);
         sdv[i] = SIG * d->durations[i];
     }
+    // ATRISK saturation thresholds of every divisor it uses
+    std::vector<double> atp1(K, 0.0), atp0(K, 0.0), atq2((size_t)K * K * 2, 0.0), atq1((size_t)K * 2, 0.0);
+    for (int a = 0; a < (int)cval.size(); ++a) {
+        const double sa = SIG * cval[a];
+        if (sa > 0.0) {
+            atp1[a] = ncdf_one_from(sa);
+            atp0[a] = ncdf_zero_to(sa);
+        }
+        for (int b = 0; b < (int)cval.size(); ++b) {
+            const double r = pw2[(size_t)a * K + b];
+            if (r > 0.0) {
+                atq2[((size_t)a * K + b) * 2] = ncdf_one_from(r);
+                atq2[((size_t)a * K + b) * 2 + 1] = ncdf_zero_to(r);
+            }
+        }
+        if (pw1[a] > 0.0) {
+            atq1[2 * a] = ncdf_one_from(pw1[a]);
+            atq1[2 * a + 1] = ncdf_zero_to(pw1[a]);
+        }
+    }
 
     const size_t nn = (size_t)ld * ld;
     const size_t np = (size_t)std::max(8, (n + 7) / 8 * 8);   // customer arrays padded for 4-wide reads
@@ -1133,6 +1195,8 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
                  o_close = take(np * 8), o_dur = take(np * 8), o_d2d = take(np * 8),
                  o_score = take(np * 8), o_durrob = take(np * 8), o_sdv = take(np * 8),
                  o_cls = take(n * 4 + 8), o_pw2 = take((size_t)K * K * 8), o_pw1 = take(K * 8),
+                 o_atp1 = take(K * 8), o_atp0 = take(K * 8), o_atq2 = take((size_t)K * K * 16),
+                 o_atq1 = take((size_t)K * 16),
                  o_zp = take(10 * 8), o_feat = take((size_t)n * 5 * 8 + 8), o_ok = take(n * 4 + 8);
     char *base = nullptr;
     HIP_TRY(hipMalloc((void **)&base, off));
@@ -1152,6 +1216,10 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     HIP_TRY(up(o_cls, cls.data(), n * 4));
     HIP_TRY(up(o_pw2, pw2.data(), (size_t)K * K * 8));
     HIP_TRY(up(o_pw1, pw1.data(), K * 8));
+    HIP_TRY(up(o_atp1, atp1.data(), K * 8));
+    HIP_TRY(up(o_atp0, atp0.data(), K * 8));
+    HIP_TRY(up(o_atq2, atq2.data(), (size_t)K * K * 16));
+    HIP_TRY(up(o_atq1, atq1.data(), (size_t)K * 16));
 
     DevInst &D = I->dev;
     D.n = n;
@@ -1174,6 +1242,10 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     D.dcls = (cip)(base + o_cls);
     D.pw2 = cd(o_pw2);
     D.pw1 = cd(o_pw1);
+    D.atp1 = cd(o_atp1);
+    D.atp0 = cd(o_atp0);
+    D.atq2 = cd(o_atq2);
+    D.atq1 = cd(o_atq1);
     D.zp = cd(o_zp);
 
     {

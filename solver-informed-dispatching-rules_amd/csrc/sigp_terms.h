@@ -43,6 +43,14 @@ struct DevInst {
     cip dcls;    // duration class of each customer
     cdp pw2;     // K x K: pow(sd_c*sd_c + sd_j*sd_j, 0.5)
     cdp pw1;     // K: pow(0.0 + sd_j*sd_j, 0.5)
+    // ATRISK saturation thresholds (engine/sim_terms.hpp:221-252): for a
+    // divisor s, norm_cdf(d / s) is exactly 1 for d >= hi(s) and exactly 0
+    // for d <= lo(s) -- erf returns +-1 for arguments of magnitude >= 6
+    // and (d / s) * 0.7071067811865476 is monotone in d.  Per duration class
+    // for s = 0.2 * dur (atp1 / atp0), per pw2 entry (atq2: K x K pairs hi,
+    // lo) and per pw1 entry (atq1: K pairs hi, lo).
+    cdp atp1, atp0;
+    cdp atq2, atq1;
     cdp zp;      // 10 z-statistics
 };
 
