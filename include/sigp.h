@@ -399,8 +399,17 @@ sigp_status sigp_gp_step(sigp_gp *gp, const double *fitness);
  * and then population - elitism offspring outside the elites' slots:
  * phenotypes not seen at an earlier slot of the run first (phenotype =
  * FNV-1a 64-bit hash of the ranks, one byte each), then higher score, then
- * lower slot. */
+ * lower slot.  Until a network is attached, and after net = NULL (sit may
+ * then be NULL) detaches it, such a run takes plain steps: the next parents
+ * are sub-step 0's offspring, the gp_step of the run's own stream. */
 sigp_status sigp_gp_set_surrogate(sigp_gp *gp, sigp_mlp *net, const sigp_situations *sit);
+/* Phenotypes of the current population (no counterpart in the reference):
+ * ranks (n_runs*population x n_sit x n_cand int32, as sigp_phenotype) of
+ * every tree on `sit`, computed from the device's own compiled trees.  With
+ * sigp_gp_init and sigp_gp_evaluate it builds a surrogate's labelled set --
+ * random rules, their exact mean rewards and their phenotypes -- without the
+ * rules leaving the device. */
+sigp_status sigp_gp_phenotype(sigp_gp *gp, const sigp_situations *sit, int32_t *ranks);
 /* The whole loop on the device: for g < generations, evaluate on rows
  * g*n_runs*T .. (n_rows = generations*n_runs*T) and step, except after the
  * last unless flags has SIGP_GP_STEP_LAST (a loop run in several calls).

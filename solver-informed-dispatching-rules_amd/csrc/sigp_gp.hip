@@ -742,6 +742,9 @@ struct sigp_gp {
     float *pred = nullptr;                      // screened steps: predicted rewards
     sigp_batch *bt = nullptr;         // the parents' evaluation
     sigp_batch *obt = nullptr;        // the offspring's phenotype
+    sigp_batch *pbt = nullptr;        // the parents' phenotype (sigp_gp_phenotype; made on first use)
+    int KL = 1, SL = 1;               // the batches' value-store and stack rows
+    bool rmod = false, rare = false;  // and their kernel variants
     sigp_mlp *net = nullptr;          // the screen (not owned)
     char *sbuf = nullptr;             // the decision situations
     sigp_device_situations sit{};
@@ -784,14 +787,16 @@ static sigp_status launch_evaluation(sigp_gp *g) {
 
 // One step of every run: breeding, then (screened) the offspring's
 // compilation, phenotypes, predicted rewards and the choice of the next
-// parents, then the new parents' analysis.
+// parents, then the new parents' analysis.  A loop created for screening
+// with no network attached takes sub-step 0's offspring, which is the plain
+// gp_step on the run's own stream.
 static sigp_status launch_step(sigp_gp *g) {
     const GpParams &p = g->prm;
     const size_t lds = (size_t)p.pop * (sizeof(double) + 4 + 1);
     plan_kernel<<<g->runs * p.subs, 64, lds, g->stream>>>(p, g->mt, g->fit, parents_of(g), g->rec, g->grow);
     assemble_kernel<<<dim3(p.n_off, g->runs), 64, 0, g->stream>>>(p, g->rec, g->ptok, g->grow, g->otok, g->olen);
     const int32_t *sel = nullptr;
-    if (p.subs > 1) {
+    if (p.subs > 1 && g->net) {
         analyze_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(p, g->oset, g->mask);
         sigp_status st = sigp_internal_phenotype_device(g->obt, g->sit, g->ranks, g->stream);
         if (st != SIGP_OK) return st;
@@ -993,6 +998,10 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
         lc.mapping = SIGP_MAP_CANDIDATES_2;
         lc.waves_per_item = 0;
     }
+    g->KL = KL;
+    g->SL = SL;
+    g->rmod = rmod;
+    g->rare = rare;
     if (status == SIGP_OK)
         status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, lc, &g->bt, g->rep);
     if (status == SIGP_OK && screened)
@@ -1005,9 +1014,10 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     return SIGP_OK;
 }
 
-extern "C" sigp_status sigp_gp_set_surrogate(sigp_gp *g, sigp_mlp *net, const sigp_situations *sit) {
-    if (!g || !net || !sit) return gp_fail("NULL argument");
-    if (g->prm.subs < 2) return gp_fail("a screen needs offspring > population");
+// The decision situations in one device allocation (*buf, freed by the
+// caller), checked against the instance.
+static sigp_status upload_situations(const sigp_gp *g, const sigp_situations *sit, char **buf,
+                                     sigp_device_situations *ds) {
     const int S = sit->n_sit, C = sit->n_cand, n = g->n;
     if (S < 1 || C < 2 || (long long)S * C != SIGP_MLP_IN)
         return gp_fail("need n_sit x n_cand = " + std::to_string(SIGP_MLP_IN) + " and n_cand >= 2");
@@ -1023,29 +1033,71 @@ extern "C" sigp_status sigp_gp_set_surrogate(sigp_gp *g, sigp_mlp *net, const si
             if (sit->cand[(size_t)q * C + j] < 0 || sit->cand[(size_t)q * C + j] >= n)
                 return gp_fail("candidate out of range");
     }
+    const size_t o_cur = 0, o_rem = 256 * ((S * 4 + 255) / 256), o_cand = o_rem + 256 * ((S * 4 + 255) / 256),
+                 o_ct = o_cand + 256 * (((size_t)S * C * 4 + 255) / 256), o_vis = o_ct + 256 * ((S * 8 + 255) / 256),
+                 total = o_vis + words.size() * 4;
+    char *b = nullptr;
+    GP_TRY(hipMalloc((void **)&b, total));
+    *buf = b;
+    GP_TRY(hipMemcpy(b + o_cur, sit->cur, S * 4, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(b + o_rem, sit->remaining, S * 4, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(b + o_cand, sit->cand, (size_t)S * C * 4, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(b + o_ct, sit->ct, S * 8, hipMemcpyHostToDevice));
+    GP_TRY(hipMemcpy(b + o_vis, words.data(), words.size() * 4, hipMemcpyHostToDevice));
+    ds->n_sit = S;
+    ds->n_cand = C;
+    ds->cur = (const int32_t *)(b + o_cur);
+    ds->rem = (const int32_t *)(b + o_rem);
+    ds->cand = (const int32_t *)(b + o_cand);
+    ds->ct = (const double *)(b + o_ct);
+    ds->vis = (const uint32_t *)(b + o_vis);
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_set_surrogate(sigp_gp *g, sigp_mlp *net, const sigp_situations *sit) {
+    if (!g) return gp_fail("NULL argument");
+    if (g->prm.subs < 2) return gp_fail("a screen needs offspring > population");
     GP_TRY(hipStreamSynchronize(g->stream));
+    if (!net) {                       // detach: the next steps are plain gp_steps
+        g->net = nullptr;
+        return SIGP_OK;
+    }
+    if (!sit) return gp_fail("NULL argument");
     if (g->sbuf) {
         (void)hipFree(g->sbuf);
         g->sbuf = nullptr;
     }
-    const size_t o_cur = 0, o_rem = 256 * ((S * 4 + 255) / 256), o_cand = o_rem + 256 * ((S * 4 + 255) / 256),
-                 o_ct = o_cand + 256 * (((size_t)S * C * 4 + 255) / 256), o_vis = o_ct + 256 * ((S * 8 + 255) / 256),
-                 total = o_vis + words.size() * 4;
-    GP_TRY(hipMalloc((void **)&g->sbuf, total));
-    GP_TRY(hipMemcpy(g->sbuf + o_cur, sit->cur, S * 4, hipMemcpyHostToDevice));
-    GP_TRY(hipMemcpy(g->sbuf + o_rem, sit->remaining, S * 4, hipMemcpyHostToDevice));
-    GP_TRY(hipMemcpy(g->sbuf + o_cand, sit->cand, (size_t)S * C * 4, hipMemcpyHostToDevice));
-    GP_TRY(hipMemcpy(g->sbuf + o_ct, sit->ct, S * 8, hipMemcpyHostToDevice));
-    GP_TRY(hipMemcpy(g->sbuf + o_vis, words.data(), words.size() * 4, hipMemcpyHostToDevice));
-    g->sit.n_sit = S;
-    g->sit.n_cand = C;
-    g->sit.cur = (const int32_t *)(g->sbuf + o_cur);
-    g->sit.rem = (const int32_t *)(g->sbuf + o_rem);
-    g->sit.cand = (const int32_t *)(g->sbuf + o_cand);
-    g->sit.ct = (const double *)(g->sbuf + o_ct);
-    g->sit.vis = (const uint32_t *)(g->sbuf + o_vis);
+    g->net = nullptr;
+    const sigp_status st = upload_situations(g, sit, &g->sbuf, &g->sit);
+    if (st != SIGP_OK) return st;
     g->net = net;
     return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_phenotype(sigp_gp *g, const sigp_situations *sit, int32_t *ranks) {
+    if (!g || !sit || !ranks) return gp_fail("NULL argument");
+    if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
+    if ((long long)sit->n_sit * sit->n_cand != SIGP_MLP_IN)
+        return gp_fail("need n_sit x n_cand = " + std::to_string(SIGP_MLP_IN));
+    sigp_status st = SIGP_OK;
+    if (!g->pbt &&
+        (st = make_rules_batch(g, g->pset, g->P, nullptr, g->KL, g->SL, g->rmod, g->rare, 1, g->P,
+                               sigp_launch_config{}, &g->pbt)) != SIGP_OK)
+        return st;
+    char *sb = nullptr, *out = nullptr;
+    sigp_device_situations ds{};
+    const size_t bytes = (size_t)g->P * SIGP_MLP_IN * 4;
+    st = upload_situations(g, sit, &sb, &ds);
+    if (st == SIGP_OK && hipMalloc((void **)&out, bytes) != hipSuccess)
+        st = sigp_internal_fail(SIGP_OOM, "phenotype output allocation failed");
+    if (st == SIGP_OK) st = sigp_internal_phenotype_device(g->pbt, ds, (int32_t *)out, g->stream);
+    if (st == SIGP_OK && (hipMemcpyAsync(ranks, out, bytes, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                          hipStreamSynchronize(g->stream) != hipSuccess))
+        st = sigp_internal_fail(SIGP_DEVICE_ERROR, "phenotype failed");
+    (void)hipStreamSynchronize(g->stream);
+    if (out) (void)hipFree(out);
+    if (sb) (void)hipFree(sb);
+    return st;
 }
 
 extern "C" sigp_status sigp_gp_destroy(sigp_gp *g) {
@@ -1053,6 +1105,7 @@ extern "C" sigp_status sigp_gp_destroy(sigp_gp *g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     if (g->bt) sigp_batch_destroy(g->bt);
     if (g->obt) sigp_batch_destroy(g->obt);
+    if (g->pbt) sigp_batch_destroy(g->pbt);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     if (g->dbuf) (void)hipFree(g->dbuf);
     if (g->sbuf) (void)hipFree(g->sbuf);
@@ -1074,7 +1127,6 @@ extern "C" sigp_status sigp_gp_init(sigp_gp *g) {
 
 static sigp_status check_step(const sigp_gp *g) {
     if (!g->ready) return gp_fail("no population: call sigp_gp_init first");
-    if (g->prm.subs > 1 && !g->net) return gp_fail("screened steps need sigp_gp_set_surrogate first");
     return SIGP_OK;
 }
 

@@ -73,6 +73,14 @@ class RunResult:
         return float(self.generation_ms[-1])
 
 
+def _situations(sit):
+    """The C struct over `sit`'s arrays, and the arrays it points into."""
+    arrs = [np.ascontiguousarray(sit.cur, dtype=np.int32), np.ascontiguousarray(sit.ct, dtype=np.float64),
+            np.ascontiguousarray(sit.remaining, dtype=np.int32),
+            np.ascontiguousarray(sit.visited, dtype=np.uint8), np.ascontiguousarray(sit.cand, dtype=np.int32)]
+    return _Situations(sit.n_sit, sit.n_cand, *[a.ctypes.data for a in arrs]), arrs
+
+
 class DeviceGP:
     """Populations of `len(seeds)` runs on `ctx`'s instance; run r uses
     Mt19937(seeds[r]) and reads scenario rows r*T .. r*T+T-1 of each
@@ -135,15 +143,27 @@ class DeviceGP:
         except Exception:
             pass
 
-    def set_surrogate(self, net, sit) -> None:
+    def set_surrogate(self, net, sit=None) -> None:
         """Screen offspring with `net` (sigp.surrogate.Mlp, kept alive here)
-        on the decision situations `sit` (64 x 8)."""
-        arrs = [np.ascontiguousarray(sit.cur, dtype=np.int32), np.ascontiguousarray(sit.ct, dtype=np.float64),
-                np.ascontiguousarray(sit.remaining, dtype=np.int32),
-                np.ascontiguousarray(sit.visited, dtype=np.uint8), np.ascontiguousarray(sit.cand, dtype=np.int32)]
-        s = _Situations(sit.n_sit, sit.n_cand, *[a.ctypes.data for a in arrs])
+        on the decision situations `sit` (64 x 8).  net = None detaches the
+        screen: the next steps are plain gp_steps (sub-step 0's offspring),
+        as they are before a network is first attached."""
+        if net is None:
+            check(lib().sigp_gp_set_surrogate(self._h, None, None))
+            self._net = None
+            return
+        s, keep = _situations(sit)
         check(lib().sigp_gp_set_surrogate(self._h, net._h, C.byref(s)))
         self._net = net
+
+    def phenotype(self, sit) -> np.ndarray:
+        """int32 ranks (runs * pop, n_sit, n_cand) of the current population
+        on the decision situations `sit`, as sigp.surrogate.phenotype of the
+        same trees."""
+        s, keep = _situations(sit)
+        out = np.zeros((self.runs * self.pop, sit.n_sit, sit.n_cand), dtype=np.int32)
+        check(lib().sigp_gp_phenotype(self._h, C.byref(s), _ptr(out)))
+        return out
 
     def set_sampler(self, run_seeds: Sequence[int], mode: str = "gamma", sigma: float = 0.2, shape: float = 4.0,
                     gen_stride: int = 1, gen_offset: int = 0) -> None:

@@ -21,6 +21,10 @@
    compared with plain GP's whole run and with plain GP's own time to that
    band, each with and without the network's setup (labelling, phenotypes,
    training), which the runs sharing the network split between them.
+   With --online there is no setup: each surrogate run starts with plain
+   steps and trains its own network from the phenotypes and exact fitness
+   of the populations it evaluates anyway (run_device_gp_online), and that
+   time is part of the run's.
 
 Prints one JSON object.  Every reward is computed by the exact simulator.
 """
@@ -83,16 +87,54 @@ class Scorer:
         return X
 
 
-def train_network(sc: Scorer, n_rules: int, epochs: int, seed: int):
-    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
-    trees = ramped_half_and_half(tc, random.Random(seed), n_rules)
-    scen = sc.train_scenarios(0)
+def device_label_set(sc: Scorer, n_rules: int, scen: np.ndarray, seed: int, pop_max: int = 4096):
+    """A labelled set built on the device: ceil(n_rules / pop_max) GP
+    populations initialised by ramped half-and-half (run r seeded
+    seed * 1,000,003 + r), each tree's exact mean over `scen` and its
+    phenotype, computed from the device's own compiled trees.  Returns
+    (features, labels, label seconds, phenotype seconds)."""
+    from .gp_device import DeviceGP
+    runs = -(-n_rules // pop_max)
+    pop = -(-n_rules // runs)
     t0 = time.perf_counter()
-    y = sc.exact(trees, scen)
+    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
+    ec = EvolutionConfig(population_size=pop, max_depth=sc.depth)
+    gp = DeviceGP(sc.ctx, tc, ec, seeds=[seed * 1_000_003 + r for r in range(runs)], T=scen.shape[0],
+                  term_mask=sc.mask)
+    gp.init()
+    y = gp.evaluate(np.tile(scen, (runs, 1))).reshape(-1)
     t_label = time.perf_counter() - t0
     t0 = time.perf_counter()
-    X = sc.features(trees)
+    X = features(gp.phenotype(sc.sit))
     t_pheno = time.perf_counter() - t0
+    gp.close()
+    return X, y, t_label, t_pheno
+
+
+def train_network(sc: Scorer, n_rules: int, epochs: int, seed: int, label_scenarios: int = 0,
+                  on_device: bool = True):
+    """The network trained on random rules (ramped half-and-half) labelled by
+    their exact mean over the first `label_scenarios` (0: all 50) label
+    scenarios; held out: the last 20 % of them.  On the device the rules are
+    generated, compiled, labelled and phenotyped there (device_label_set,
+    n_rules rounded up to whole populations); on the host they are
+    generated and compiled in Python.  Either way the label time counts the
+    rules' generation."""
+    scen = sc.train_scenarios(0)
+    if label_scenarios:
+        scen = np.ascontiguousarray(scen[:label_scenarios])
+    if on_device:
+        X, y, t_label, t_pheno = device_label_set(sc, n_rules, scen, seed)
+        n_rules = y.shape[0]
+    else:
+        t0 = time.perf_counter()
+        tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
+        trees = ramped_half_and_half(tc, random.Random(seed), n_rules)
+        y = sc.exact(trees, scen)
+        t_label = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        X = sc.features(trees)
+        t_pheno = time.perf_counter() - t0
     mu, sd = float(y.mean()), float(y.std()) or 1.0
     ntr = int(0.8 * n_rules)
     net = Mlp(seed=seed)
@@ -101,7 +143,8 @@ def train_network(sc: Scorer, n_rules: int, epochs: int, seed: int):
                        lr=1e-3, seed=seed + 1)
     t_train = time.perf_counter() - t0
     pred = net.predict(X[ntr:]) * sd + mu
-    stats = {"rules": n_rules, "train_rules": ntr, "epochs": epochs,
+    stats = {"rules": n_rules, "train_rules": ntr, "epochs": epochs, "label_scenarios": int(scen.shape[0]),
+             "labelled_on": "device" if on_device else "host",
              "spearman": round(spearman(pred, y[ntr:]), 4),
              "mae": round(float(np.abs(pred - y[ntr:]).mean()), 3),
              "reward_sd": round(sd, 3), "final_loss": round(float(losses[-1]), 5),
@@ -158,6 +201,73 @@ def run_device_gp(sc: Scorer, net, generations: int, pop: int, breed: int, seed:
             "generation_ms": [round(float(v), 1) for v in gms]}
 
 
+def run_device_gp_online(sc: Scorer, generations: int, pop: int, breed: int, seed: int, label: str,
+                         train_at=(2, 4, 8, 16), epochs: int = 20, cap: int = 16384, chunk: int = 5,
+                         launch=None):
+    """Surrogate-assisted GP whose network learns inside the run, from the
+    run's own exact evaluations: until generation train_at[0] the loop takes
+    plain steps; before each generation up to train_at[-1] the population's
+    phenotypes are taken, and after its evaluation they are kept with their
+    exact fitness; at each generation of `train_at` the network (a fresh one
+    at the first) trains on the latest `cap` of them, and from the first on it
+    screens breed = k x pop offspring per generation.  No rule is simulated
+    for the network alone.  The time at the end of each generation is the
+    device time plus the host time of the phenotypes and the training, both
+    spent inside the run.  Returns what run_device_gp returns, plus the
+    training record."""
+    from .gp_device import DeviceGP
+    tc = TreeConfig(list(VOCAB), list(FUNCTIONS), sc.depth, 2)
+    ec = EvolutionConfig(population_size=pop, max_depth=sc.depth)
+    gp = DeviceGP(sc.ctx, tc, ec, seeds=[seed], T=sc.n_train, term_mask=sc.mask, offspring=breed, **(launch or {}))
+    gp.init()
+    net = Mlp(seed=seed)
+    X, Y = [], []
+    best, gms, trained = [], [], []
+    t_dev, t_host = 0.0, 0.0
+    last = max(train_at)
+    g = 0
+    while g < generations:
+        if g < last:
+            t0 = time.perf_counter()
+            Xg = features(phenotype(sc.ctx, gp.packed(), sc.mask, sc.sit))
+            t_host += 1000.0 * (time.perf_counter() - t0)
+            res = gp.run(sc.train_scenarios(g), generations=1, graph=False, step_last=g + 1 < generations)
+            X.append(Xg)
+            Y.append(res.fitness[0, 0].astype(np.float32))
+            k = 1
+        else:
+            k = min(chunk, generations - g)
+            rows = np.concatenate([sc.train_scenarios(q) for q in range(g, g + k)])
+            res = gp.run(rows, generations=k, graph=True, step_last=g + k < generations)
+        best += [res.best[q][0] for q in range(k)]
+        base = t_dev
+        for q in range(k):
+            gms.append(base + float(res.generation_ms[q]) + t_host)
+        t_dev = base + float(res.generation_ms[-1])
+        g += k
+        if g in train_at and g < generations:
+            t0 = time.perf_counter()
+            Xa = np.concatenate(X)[-cap:]
+            Ya = np.concatenate(Y)[-cap:]
+            mu, sd = float(Ya.mean()), float(Ya.std()) or 1.0
+            losses = net.train(Xa, ((Ya - mu) / sd).astype(np.float32), epochs=epochs, batch=256, lr=1e-3,
+                               seed=seed + g)
+            if not trained:
+                gp.set_surrogate(net, sc.sit)
+            dt = 1000.0 * (time.perf_counter() - t0)
+            t_host += dt
+            trained.append({"generation": g, "rows": int(Xa.shape[0]), "ms": round(dt, 1),
+                            "final_loss": round(float(losses[-1]), 5)})
+            _log("  %s: network trained at generation %d on %d rules in %.0f ms" % (label, g, Xa.shape[0], dt))
+    held = sc.exact(best, sc.test)
+    _log("%s: %d generations in %.1f ms (host %.0f ms of it); held-out of the last best %.2f"
+         % (label, generations, gms[-1], t_host, float(held[-1])))
+    gp.close()
+    net.close()
+    return {"held_out_best_per_generation": [round(float(v), 3) for v in held],
+            "generation_ms": [round(float(v), 1) for v in gms], "host_ms": round(t_host, 1), "training": trained}
+
+
 def reach(curve, ms, target):
     """First generation whose held-out reward is within 0.5 % of `target`,
     and the device time at its end."""
@@ -186,6 +296,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--rules", type=int, default=100_000)
     ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--label-scenarios", type=int, default=0,
+                    help="scenarios per labelled rule (0: the 50 training scenarios)")
+    ap.add_argument("--labels-on-host", action="store_true",
+                    help="generate and compile the labelled rules in Python instead of on the device")
     ap.add_argument("--pool", type=int, default=8192)
     ap.add_argument("--generations", type=int, default=50)
     ap.add_argument("--config", choices=["clustered", "large"], default="clustered")
@@ -195,6 +309,9 @@ def main(argv=None):
     ap.add_argument("--breed", type=int, default=0, help="offspring bred per generation (0: 8 x population)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--gp-runs", type=int, default=5, help="GP seed pairs (plain, surrogate) with one network")
+    ap.add_argument("--online", action="store_true",
+                    help="train each surrogate run's network inside the run from its own exact evaluations "
+                         "(no labelling setup) instead of one network trained beforehand")
     a = ap.parse_args(argv)
     if a.config == "large":
         # n=288 clustered + uniform, m=4, windows U[20,200], Gamma(2), rules <= 127 nodes
@@ -212,24 +329,37 @@ def main(argv=None):
     if a.waves:
         launch = dict(waves_per_item=a.waves, mapping={"scenarios": 1, "candidates": 2,
                                                        "candidates2": 3}[a.mapping])
-    net, norm, stats = train_network(sc, a.rules, a.epochs, a.seed)
-    _log("network", json.dumps(stats))
-    gain = screening_gain(sc, net, norm, a.pool, a.seed + 100)
-    _log("screening", json.dumps(gain))
     out = {"instance": desc, "vocabulary": "NS-11 + EXTRA-18", "max_depth": sc.depth,
-           "train_scenarios": 50, "held_out_scenarios": 500, "network": stats, "screening": gain}
+           "train_scenarios": 50, "held_out_scenarios": 500}
+    if a.online:
+        net, setup_s = None, 0.0
+        out["network"] = "trained inside each surrogate run from its own exact evaluations"
+    else:
+        net, norm, stats = train_network(sc, a.rules, a.epochs, a.seed, a.label_scenarios,
+                                         on_device=not a.labels_on_host)
+        _log("network", json.dumps(stats))
+        gain = screening_gain(sc, net, norm, a.pool, a.seed + 100)
+        _log("screening", json.dumps(gain))
+        out.update(network=stats, screening=gain)
+        setup_s = sum(stats["seconds"].values())
     if a.generations > 0:
         # warm-up of the device loops (code objects, graph instantiation)
         run_device_gp(sc, None, 2, 64, 64, a.seed + 7, "warm-up")
-        run_device_gp(sc, net, 2, 64, 512, a.seed + 7, "warm-up")
-        setup_s = sum(stats["seconds"].values())
+        if a.online:
+            run_device_gp_online(sc, 3, 64, 512, a.seed + 7, "warm-up", train_at=(1, 2))
+        else:
+            run_device_gp(sc, net, 2, 64, 512, a.seed + 7, "warm-up")
         runs = []
         for i in range(a.gp_runs):
             gseed = a.seed + 7 + i
             plain = run_device_gp(sc, None, a.generations, a.pop, a.pop, gseed, "plain seed %d" % gseed,
                                   launch=launch)
-            surr = run_device_gp(sc, net, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed,
-                                 launch=launch)
+            if a.online:
+                surr = run_device_gp_online(sc, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed,
+                                            launch=launch)
+            else:
+                surr = run_device_gp(sc, net, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed,
+                                     launch=launch)
             target = plain["held_out_best_per_generation"][-1]
             pg, pms = reach(plain["held_out_best_per_generation"], plain["generation_ms"], target)
             sgen, sms = reach(surr["held_out_best_per_generation"], surr["generation_ms"], target)
@@ -243,7 +373,8 @@ def main(argv=None):
                  "plain run %.0f ms, setup share %.0f ms"
                  % (gseed, target, pg, pms, sgen, sms if sms is not None else float("nan"), total, share))
         out["gp"] = {"generations": a.generations, "pop": a.pop, "breed": a.breed, "runs": runs,
-                     "network_setup_s": round(setup_s, 3), "runs_sharing_the_network": a.gp_runs}
+                     "network_setup_s": round(setup_s, 3),
+                     "runs_sharing_the_network": 1 if a.online else a.gp_runs}
         for key in ("time_ratio", "time_ratio_with_setup", "time_ratio_to_plain_reach",
                     "time_ratio_to_plain_reach_with_setup"):
             vals = [r[key] if r[key] is not None else float("inf") for r in runs]

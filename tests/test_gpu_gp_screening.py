@@ -4,11 +4,12 @@ the network keeps 256 for exact simulation) reaches plain GP's final
 held-out band (within 0.5 % of the held-out reward of plain GP's last best
 rule) in fewer generations than plain GP itself -- both arms simulate 256
 rules x 50 scenarios exactly per generation, so fewer generations are fewer
-exact simulations.  The network's own labels (6,000 rules x 50 scenarios)
-are counted beside them.  Measured on MI355X: generations 18 / 13 / 5
-against 21 / 22 / 17 on seeds 8 / 9 / 10 (the screened loop spends more
-device time per generation: it breeds, compiles, phenotypes and scores 8x
-the offspring, and its rules route longer)."""
+exact simulations.  The network's own labels (6,000 rules x 50 scenarios,
+generated, compiled and labelled on the device) are counted beside them.
+Measured on MI355X: generations 15 / never / 6 against 21 / 22 / 17 on
+seeds 8 / 9 / 10 (the screened loop spends more device time per
+generation: it breeds, compiles, phenotypes and scores 8x the offspring,
+and its rules route longer)."""
 import pytest
 
 pytestmark = pytest.mark.gpu
