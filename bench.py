@@ -500,6 +500,10 @@ def main():
     ap.add_argument("--mapping", choices=sorted(MAPPINGS), default="scenarios",
                     help="lane mapping used with --waves (candidates2: 2 simulations per wavefront, "
                          "--waves 1)")
+    ap.add_argument("--protocol", choices=["vehicles", "rr"], default="vehicles",
+                    help="plain / large / batched: vehicles = sim_one (the reference's V0, the "
+                         "headline); rr = sim_one_rr, the round-robin ablation arm (no roofline: the "
+                         "committed work models count V0)")
     ap.add_argument("--no-scenario-order", action="store_true",
                     help="scenario lanes in index order (sigp_batch_order_scenarios off)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -543,8 +547,11 @@ def main():
     name, n_cu, clk = device_info()
     log("device", name, n_cu, "CUs")
     T = wl.n_scen
+    rr = args.protocol == "rr"
     if args.workload == "batched":
         ctxs = [FactoryContext(i) for i in wl.insts]
+        for c in ctxs:
+            c.protocol(rr)
         ctx = ctxs[0]
         batch = batch_multi(ctxs, wl.packed, T, wl.mask)
         n_rows = len(wl.insts) * T
@@ -552,6 +559,7 @@ def main():
         P = len(wl.insts) * wl.packed.n_rules     # (instance, rule) pairs
     else:
         ctx = FactoryContext(wl.inst)
+        ctx.protocol(rr)
         batch = ctx.batch(wl.packed, T, wl.mask, group_size=wl.group_size or None)
         n_rows = wl.sampled.shape[0]
         host_rows = wl.sampled
@@ -600,7 +608,7 @@ def main():
 
     roof = None
     # the work model and the reference's CPU timing exist for the headline workload
-    model = None if args.no_roofline else work_model(args.workload, wl)
+    model = None if (args.no_roofline or rr) else work_model(args.workload, wl)
     if model is not None:
         # The simulation kernel(s) alone (no per-rule mean kernel), same
         # stream; a multi-instance batch forks its size classes onto side
@@ -633,14 +641,14 @@ def main():
         del src, dst
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(wl, ctx.zp) if args.workload == "plain" else None
+        cpu = cpu_baseline(wl, ctx.zp) if (args.workload == "plain" and not rr) else None
         log("cpu baseline", cpu)
     out = {
         "metric": "rule x scenario sims/sec", "value": round(value, 1), "unit": "sims/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": describe(args.workload, wl),
+        "config": {"workload": describe(args.workload, wl) + ("; round-robin protocol (sim_one_rr)" if rr else ""),
                    "rules": P, "scenarios": T,
                    "n": wl.inst.n if hasattr(wl, "inst") else None,
                    "m": wl.inst.m if hasattr(wl, "inst") else None,

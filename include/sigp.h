@@ -238,6 +238,25 @@ sigp_status sigp_instance_skip_repeats(sigp_instance *inst, int32_t enable);
  * ones sigp_evaluate creates internally. */
 sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_launch_config *cfg);
 
+/* Simulation protocol.  SIGP_PROTOCOL_VEHICLES is sim_one
+ * (engine/sim_core.hpp:207-238, the reference's V0 and its only GPU path):
+ * vehicle 0 routes until pick_best finds no candidate, then vehicle 1, ...
+ * SIGP_PROTOCOL_ROUND_ROBIN is sim_one_rr (engine/sim_core.hpp:241-274, the
+ * reference's V2 ablation arm, CPU only there): the open vehicles take one
+ * decision each in turn, 0 .. m-1, a vehicle closing when it finds no
+ * candidate; pick_best still sees "vehicles left" = m - v - 1 for vehicle v.
+ * The round-robin arm needs m <= SIGP_RR_MAX_M on every instance of the
+ * batch (each lane keeps every vehicle's route state in LDS).
+ * sigp_instance_protocol sets the protocol of batches created later on the
+ * instance, including the ones sigp_evaluate and sigp_evaluate_noalloc
+ * create; a multi-instance batch needs its instances to agree.
+ * sigp_batch_protocol switches an existing batch. */
+#define SIGP_PROTOCOL_VEHICLES 0
+#define SIGP_PROTOCOL_ROUND_ROBIN 1
+#define SIGP_RR_MAX_M 8
+sigp_status sigp_instance_protocol(sigp_instance *inst, int32_t protocol);
+sigp_status sigp_batch_protocol(sigp_batch *batch, int32_t protocol);
+
 /* Number of rules and (rule, scenario) pairs in a batch. */
 sigp_status sigp_batch_shape(const sigp_batch *batch, int32_t *n_rules, int64_t *n_pairs);
 

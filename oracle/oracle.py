@@ -166,6 +166,35 @@ class Oracle:
         sm, mk, z = _f64(sampled_row), np.ascontiguousarray(mask, dtype=np.int64), _f64(zp)
         return self.L.orc_sim_one_rr(C.byref(s.st), _p(o), _p(a), len(o), _p(sm), _p(mk), _p(z))
 
+    def evaluate_rr(self, inst, pk, sampled, N, masks, zp, group_size=None, mask_group=None,
+                    threads: int = 1) -> np.ndarray:
+        """out[p*N+s] = orc_sim_one_rr of rule p on scenario row
+        (p / group_size) * N + s with mask row p / mask_group: the grouped
+        index maths of evaluate, under the round-robin protocol.  Rules run
+        on `threads` host threads (ctypes releases the GIL)."""
+        from concurrent.futures import ThreadPoolExecutor
+        st = _Staged(inst)
+        P = pk.n_rules
+        gs = group_size or max(P, 1)
+        mg = mask_group or max(P, 1)
+        S = _f64(sampled).reshape(-1, max(inst.n, 1))
+        mk = np.ascontiguousarray(np.atleast_2d(masks), dtype=np.int64)
+        z = _f64(zp)
+        ops, args = _i32(pk.ops), _i32(pk.args)
+        out = np.zeros(P * N)
+
+        def rule(p):
+            o, L = int(pk.offsets[p]), int(pk.lengths[p])
+            op, ar = ops[o:o + L], args[2 * o:2 * (o + L)]
+            row0 = (p // gs) * N
+            for s in range(N):
+                out[p * N + s] = self.L.orc_sim_one_rr(C.byref(st.st), _p(op), _p(ar), L, _p(S[row0 + s]),
+                                                       _p(mk[p // mg]), _p(z))
+
+        with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+            list(ex.map(rule, range(P)))
+        return out
+
     def ops(self) -> int:
         return int(self.L.orc_ops_read())
 

@@ -249,6 +249,11 @@ struct ScenSpec {
     int first_only;          // candidate lanes: only record each pair's first decision in `first`
     long long n_ranks;       // scenario lanes: pairs of the launch (the grid is padded, see sim_kernel)
     const int32_t *rep;      // null, or rep[p] != p: rule p repeats rule rep[p] and is skipped
+    // Protocol: 0 = vehicle by vehicle (engine/sim_core.hpp:207-238); > 0 =
+    // round robin (engine/sim_core.hpp:241-274) with route states of up to
+    // `rr` vehicles, 2 * rr LDS rows per wavefront after the W wavefronts'
+    // value stores and stacks (see rr_rotate).
+    int rr;
 };
 
 // Workgroups are dispatched to the 8 XCDs round-robin by index.  Scenario
@@ -540,6 +545,31 @@ __device__ __forceinline__ int decide(const DevInst &I, const FusedTabs &F, cons
     return best;
 }
 
+// Round-robin protocol (engine/sim_core.hpp:241-274): the vehicles advance
+// one POI per round.  Each lane keeps the route state of every vehicle in its
+// own column of 2 * m LDS rows (clock, then current POI), all zero at the
+// start.  After vehicle `veh` has decided -- a move, or the vehicle closing
+// (its bit cleared from `open`) -- its state is saved and the next vehicle
+// still open after it, cyclically, resumes; that is the order of the
+// reference's `for v` inside `while (n_active > 0)`.  Returns false once
+// every vehicle has closed.
+__device__ __forceinline__ double *rr_rows(double *lds_rows, const RuleSet &R, int W, int wv, int rr, int lane) {
+    return lds_rows + (size_t)W * (R.KL + R.SL) * 64 + (size_t)wv * 2 * rr * 64 + lane;
+}
+__device__ __forceinline__ void rr_clear(double *rr, int m) {
+    for (int v = 0; v < 2 * m; ++v) rr[(size_t)v * 64] = 0.0;
+}
+__device__ __forceinline__ bool rr_rotate(double *rr, uint32_t open, int &veh, int &cur, double &ct) {
+    rr[(size_t)(2 * veh) * 64] = ct;
+    rr[(size_t)(2 * veh + 1) * 64] = (double)cur;
+    if (open == 0) return false;
+    const uint32_t later = open & (~0u << (veh + 1));   // veh < SIGP_RR_MAX_M
+    veh = later ? __builtin_ctz(later) : __builtin_ctz(open);
+    ct = rr[(size_t)(2 * veh) * 64];
+    cur = (int)rr[(size_t)(2 * veh + 1) * 64];
+    return true;
+}
+
 // One workgroup = one rule x a block of 64 scenarios (engine/sim_core.hpp:
 // 207-238 per lane), simulated by W = blockDim.x / 64 cooperating
 // wavefronts: wavefront w evaluates candidates w, w + W, ... of every
@@ -595,13 +625,21 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     int cur = 0, veh = 0;
     double ct = 0.0, total = 0.0;
     bool running = active && I.m > 0;
+    double *rr = nullptr;
+    uint32_t open = 0;
+    if (S.rr) {
+        rr = rr_rows(lds_rows, R, W, wv, S.rr, lane);
+        rr_clear(rr, I.m);
+        open = (1u << I.m) - 1u;
+    }
     // A lane that has closed vehicle `veh` holds until every running lane of
     // the block has closed it; then all start the next vehicle together.
     // Lanes at the same stage of the same vehicle share most of their
     // feasible candidates, so the block walks fewer candidate bodies than
     // when each lane moves on at once (modelled on the oracle over 16 rules
     // of the n = 288 workload: 36 % fewer bodies for 18 % more decisions).
-    // A lane's own decisions are unchanged, so are its results.
+    // A lane's own decisions are unchanged, so are its results.  (Round
+    // robin never holds: its lanes rotate through the vehicles on their own.)
     bool held = false;
     while (__builtin_amdgcn_ballot_w64(running) != 0) {
         int best = -1;
@@ -627,7 +665,8 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
         }
         if (deciding) {
             if (best < 0) {
-                held = true;
+                if (S.rr) open &= ~(1u << veh);
+                else held = true;
             } else {
                 const int pb = best + 1;
                 const double ta = I.dist[(size_t)cur * I.ld + pb];
@@ -638,6 +677,7 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
                 total += I.score[best];
                 wset<NW>(vis, best);
             }
+            if (S.rr) running = rr_rotate(rr, open, veh, cur, ct);
         }
         if (__builtin_amdgcn_ballot_w64(running && !held) == 0 && running) {
             ++veh;
@@ -723,6 +763,13 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
         if (threadIdx.x == 0) S.first[pair] = -1;
         return;
     }
+    double *rr = nullptr;
+    uint32_t open = 0;
+    if (S.rr) {   // route states of the round-robin protocol (same in every thread)
+        rr = rr_rows(lds_rows, R, W, wv, S.rr, lane);
+        rr_clear(rr, I.m);
+        open = (1u << I.m) - 1u;
+    }
     while (running) {
         double bp = -1e300;
         const int rem = I.m - veh - 1;
@@ -733,7 +780,21 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
             if (threadIdx.x == 0) S.first[pair] = best;
             return;
         }
-        if (best < 0) {
+        if (S.rr) {
+            if (best < 0) {
+                open &= ~(1u << veh);
+            } else {
+                const int pb = best + 1;
+                const double ta = I.dist[(size_t)cur * I.ld + pb];
+                const double a = ct + ta;
+                const double stt = a > I.open[best] ? a : I.open[best];
+                ct = stt + samp[best];
+                cur = pb;
+                total += I.score[best];
+                wset<NW>(vis, best);
+            }
+            running = rr_rotate(rr, open, veh, cur, ct);
+        } else if (best < 0) {
             ++veh;
             cur = 0;
             ct = 0.0;
@@ -838,10 +899,17 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
     int cur = 0, veh = 0;
     double ct = 0.0, total = 0.0;
     bool running = active && I.m > 0;
+    double *rr = nullptr;
+    uint32_t open = 0;
+    if (S.rr) {   // round robin, as in sim_kernel
+        rr = rr_rows(lds_rows, R, 1, 0, S.rr, lane);
+        rr_clear(rr, I.m);
+        open = (1u << I.m) - 1u;
+    }
     // Vehicle boundaries are kept together as in sim_kernel: a simulation
     // that has closed vehicle `veh` holds until the other has closed it too
     // (modelled on 12 rules x 500 scenarios at n = 100: 10 % fewer candidate
-    // passes for 2 % more decisions).
+    // passes for 2 % more decisions).  Round robin never holds.
     bool held = false;
     while (__builtin_amdgcn_ballot_w64(running) != 0) {
         double bp = -1e300;
@@ -854,7 +922,8 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
         best = seg_argmax<SEGW>(bp, best);
         if (deciding) {
             if (best < 0) {
-                held = true;
+                if (S.rr) open &= ~(1u << veh);
+                else held = true;
             } else {
                 const int pb = best + 1;
                 const double ta = I.dist[(size_t)cur * I.ld + pb];
@@ -865,6 +934,7 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
                 total += I.score[best];
                 wset<NW>(vis, best);
             }
+            if (S.rr) running = rr_rotate(rr, open, veh, cur, ct);
         }
         if (__builtin_amdgcn_ballot_w64(running && !held) == 0 && running) {
             ++veh;
@@ -985,6 +1055,7 @@ struct sigp_instance {
     int waves_per_item = 0;         // default launch mapping of new batches (0 = per batch)
     int mapping = SIGP_MAP_AUTO;
     bool skip_repeats = false;      // new batches simulate each distinct rule once
+    int protocol = SIGP_PROTOCOL_VEHICLES;   // protocol of new batches
 };
 
 // Launch order and tuned mapping of the instances of one size class (one
@@ -1029,6 +1100,7 @@ struct sigp_batch {
     const int32_t *rep = nullptr;   // repeated rules (sigp_device_rules::rep, or rep_owned)
     int32_t *rep_owned = nullptr;   // host-detected repeats (sigp_instance_skip_repeats); filled by the batch
     int G = 1;                      // scenario groups per instance
+    int rr = 0;                     // ScenSpec::rr: 0 = vehicle by vehicle, else round robin over <= rr vehicles
 };
 
 static sigp_status check_device(int *dev_out) {
@@ -1513,6 +1585,25 @@ struct BatchPlan {
 };
 
 // Fills bt's host fields (shape, size classes, variant flags) and the plan.
+// The round-robin protocol keeps 2 * (largest m) LDS rows per wavefront.
+static sigp_status set_protocol(sigp_batch *bt, int protocol) {
+    if (protocol != SIGP_PROTOCOL_VEHICLES && protocol != SIGP_PROTOCOL_ROUND_ROBIN)
+        return fail(SIGP_INVALID_ARG, "protocol must be SIGP_PROTOCOL_VEHICLES or SIGP_PROTOCOL_ROUND_ROBIN");
+    int rr = 0;
+    if (protocol == SIGP_PROTOCOL_ROUND_ROBIN) {
+        rr = 1;
+        for (const sigp_instance *inst : bt->insts) {
+            if (inst->m > SIGP_RR_MAX_M)
+                return fail(SIGP_INVALID_ARG, "round robin simulates at most SIGP_RR_MAX_M = " +
+                                                  std::to_string(SIGP_RR_MAX_M) + " vehicles; an instance has m = " +
+                                                  std::to_string(inst->m));
+            rr = std::max(rr, inst->m);
+        }
+    }
+    bt->rr = rr;
+    return SIGP_OK;
+}
+
 static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool multi, const PreparedRules &pr,
                               int T, int group_size, sigp_batch *bt, BatchPlan &pl) {
     const int P = pr.P, K = (int)insts.size();
@@ -1528,6 +1619,11 @@ static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool mu
     bt->n_out = (long long)K * P * T;
     bt->has_rmod = pr.any_rmod;
     bt->has_rare = (pr.any_rd & RARE_SLOTS) != 0;
+    for (const sigp_instance *inst : insts)
+        if (inst->protocol != insts[0]->protocol)
+            return fail(SIGP_INVALID_ARG, "the instances of a batch must share one protocol (sigp_instance_protocol)");
+    sigp_status pst = set_protocol(bt, insts[0]->protocol);
+    if (pst != SIGP_OK) return pst;
 
     pl.slots.resize(K);
     long long samp = 0, perms = 0;
@@ -1761,6 +1857,22 @@ extern "C" sigp_status sigp_instance_skip_repeats(sigp_instance *inst, int32_t e
     return SIGP_OK;
 }
 
+extern "C" sigp_status sigp_instance_protocol(sigp_instance *inst, int32_t protocol) {
+    if (!inst) return fail(SIGP_INVALID_ARG, "instance is NULL");
+    if (protocol != SIGP_PROTOCOL_VEHICLES && protocol != SIGP_PROTOCOL_ROUND_ROBIN)
+        return fail(SIGP_INVALID_ARG, "protocol must be SIGP_PROTOCOL_VEHICLES or SIGP_PROTOCOL_ROUND_ROBIN");
+    if (protocol == SIGP_PROTOCOL_ROUND_ROBIN && inst->m > SIGP_RR_MAX_M)
+        return fail(SIGP_INVALID_ARG, "round robin simulates at most SIGP_RR_MAX_M = " +
+                                          std::to_string(SIGP_RR_MAX_M) + " vehicles");
+    inst->protocol = protocol;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_batch_protocol(sigp_batch *bt, int32_t protocol) {
+    if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
+    return set_protocol(bt, protocol);
+}
+
 extern "C" sigp_status sigp_batch_order_scenarios(sigp_batch *bt, int32_t enable, int32_t *active) {
     if (!bt) return fail(SIGP_INVALID_ARG, "batch is NULL");
     bt->order_on = enable != 0 && bt->perm != nullptr;
@@ -1798,10 +1910,13 @@ extern "C" sigp_status sigp_batch_shape(const sigp_batch *bt, int32_t *n_rules, 
 }
 
 // LDS of one workgroup: the merge arrays plus W wavefronts' value stores and
-// stacks, kept within 64 KiB.  W = 1 always fits: (61 + 31) rows * 512 B.
+// stacks (and round-robin route states), kept within 64 KiB.  W = 1 always
+// fits: (61 + 31 + 2 * SIGP_RR_MAX_M) rows * 512 B.
 static constexpr size_t LDS_LIMIT = 65536, LDS_STATIC = 4 * 64 * (8 + 4);
+static_assert(LDS_STATIC + (SIGP_N_TERMS + SIGP_MAX_STACK - 1 + 2 * SIGP_RR_MAX_M) * 512 <= LDS_LIMIT,
+              "one wavefront's LDS rows must fit");
 static size_t lds_per_wave(const sigp_batch *bt) {
-    return (size_t)(bt->rs.KL + bt->rs.SL) * 64 * sizeof(double);
+    return (size_t)(bt->rs.KL + bt->rs.SL + 2 * bt->rr) * 64 * sizeof(double);
 }
 static int fit_waves(const sigp_batch *bt, int w) {
     while (w > 1 && LDS_STATIC + (size_t)w * lds_per_wave(bt) > LDS_LIMIT) w >>= 1;
@@ -1919,6 +2034,7 @@ static sigp_status launch_unit(const sigp_batch *bt, const LaunchUnit &u, const 
     S.nblk = (bt->T + 63) / 64;
     int mapping = resolved_mapping(bt, u);
     S.rep = bt->rep;
+    S.rr = bt->rr;
     if (bt->order_on && (mapping == SIGP_MAP_SCENARIOS || mapping == SIGP_MAP_CANDIDATES_2)) {
         S.perm = bt->perm;
         S.first = bt->first;

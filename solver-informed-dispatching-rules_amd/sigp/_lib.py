@@ -12,7 +12,7 @@ __all__ = ["lib", "lib_path", "SigpError", "check", "InstanceDesc", "Rules", "La
            "GpConfig", "Sampler", "EXPORTED", "GP_MAX_NODES", "GP_TOK_FUNC", "GP_TOK_CONST",
            "SAMPLE_GAUSS", "SAMPLE_GAMMA",
            "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES",
-           "MAP_CANDIDATES_2"]
+           "MAP_CANDIDATES_2", "PROTOCOL_VEHICLES", "PROTOCOL_ROUND_ROBIN", "RR_MAX_M"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 lib_path = os.path.join(_HERE, "libsigp.so")
@@ -22,7 +22,7 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_zstats", "sigp_evaluate", "sigp_evaluate_noalloc", "sigp_batch_create", "sigp_batch_run",
             "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math", "sigp_batch_configure",
             "sigp_batch_order_scenarios", "sigp_instance_skip_repeats",
-            "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
+            "sigp_instance_protocol", "sigp_batch_protocol", "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
             "sigp_evaluate_multi", "sigp_phenotype", "sigp_mlp_create", "sigp_mlp_destroy",
             "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
             "sigp_mlp_predict", "sigp_gp_create", "sigp_gp_destroy", "sigp_gp_init", "sigp_gp_evaluate",
@@ -48,6 +48,7 @@ class InstanceDesc(C.Structure):
 
 
 MAP_AUTO, MAP_SCENARIOS, MAP_CANDIDATES, MAP_CANDIDATES_2 = 0, 1, 2, 3
+PROTOCOL_VEHICLES, PROTOCOL_ROUND_ROBIN, RR_MAX_M = 0, 1, 8
 
 
 class LaunchConfig(C.Structure):
@@ -119,6 +120,8 @@ def lib() -> C.CDLL:
         "sigp_batch_configure": ([vp, vp], C.c_int),
         "sigp_batch_order_scenarios": ([vp, C.c_int32, vp], C.c_int),
         "sigp_instance_skip_repeats": ([vp, C.c_int32], C.c_int),
+        "sigp_instance_protocol": ([vp, C.c_int32], C.c_int),
+        "sigp_batch_protocol": ([vp, C.c_int32], C.c_int),
         "sigp_batch_autotune": ([vp, vp, i64, vp, vp], C.c_int),
         "sigp_instance_configure": ([vp, vp], C.c_int),
         "sigp_batch_create_multi": ([vp, i32, vp, i32, vp, i32, i32, vp], C.c_int),

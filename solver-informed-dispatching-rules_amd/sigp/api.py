@@ -10,14 +10,14 @@ from typing import Optional, Sequence
 
 import numpy as np
 
-from ._lib import (MAP_AUTO, MAP_CANDIDATES, MAP_CANDIDATES_2, MAP_SCENARIOS, InstanceDesc,
-                   LaunchConfig, Rules, check, lib)
+from ._lib import (MAP_AUTO, MAP_CANDIDATES, MAP_CANDIDATES_2, MAP_SCENARIOS, PROTOCOL_ROUND_ROBIN,
+                   PROTOCOL_VEHICLES, RR_MAX_M, InstanceDesc, LaunchConfig, Rules, check, lib)
 from .instance import Instance
 from .rules import N_TERMS, Packed
 
 __all__ = ["FactoryContext", "Batch", "zp_of", "device_math", "device_info", "evaluate_multi",
            "batch_multi", "sample_durations", "sampler", "MAP_AUTO", "MAP_SCENARIOS", "MAP_CANDIDATES",
-           "MAP_CANDIDATES_2"]
+           "MAP_CANDIDATES_2", "RR_MAX_M"]
 
 
 def _ptr(a: Optional[np.ndarray]):
@@ -118,6 +118,11 @@ class Batch:
         cfg = LaunchConfig(waves_per_item, mapping)
         check(lib().sigp_batch_configure(self._h, C.byref(cfg)))
 
+    def protocol(self, round_robin: bool) -> None:
+        """Switch this batch between sim_one (False) and sim_one_rr (True)
+        (sigp_batch_protocol)."""
+        check(lib().sigp_batch_protocol(self._h, PROTOCOL_ROUND_ROBIN if round_robin else PROTOCOL_VEHICLES))
+
     def order_scenarios(self, enable: bool) -> bool:
         """Scenario order of scenario lanes (sigp_batch_order_scenarios); returns
         whether it is on (it needs 64 <= T <= 4096)."""
@@ -161,6 +166,14 @@ class FactoryContext:
         scenario group once and copy its rewards to the repeats
         (sigp_instance_skip_repeats)."""
         check(lib().sigp_instance_skip_repeats(self._h, 1 if enable else 0))
+
+    def protocol(self, round_robin: bool) -> None:
+        """Simulation protocol of later batches and evaluate* calls
+        (sigp_instance_protocol): False = sim_one, vehicle by vehicle
+        (engine/sim_core.hpp:207-238, the default); True = sim_one_rr, the
+        open vehicles deciding in turn (engine/sim_core.hpp:241-274;
+        m <= RR_MAX_M)."""
+        check(lib().sigp_instance_protocol(self._h, PROTOCOL_ROUND_ROBIN if round_robin else PROTOCOL_VEHICLES))
 
     def configure(self, waves_per_item: int, mapping: int = MAP_AUTO) -> None:
         """Default launch mapping of later batches and evaluate* calls."""
