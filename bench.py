@@ -410,6 +410,59 @@ def bench_trace(args):
                 {"instance": args.instance, "mean_gain": res["mean_gain"]})
 
 
+BF16_MFMA_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
+MLP_FWD_FLOP = 2 * (512 * 256 + 256 * 256)          # the two matrix layers, per sample
+MLP_TRAIN_FLOP = MLP_FWD_FLOP + 2 * (256 * 256 + 256 * 256 + 512 * 256)   # + delta1, dW2, dW1
+
+
+def bench_mlp(args):
+    """The surrogate network on the matrix cores: training on 100,000
+    phenotype-shaped rows (512 ranks / 7 on an 8-level grid) for 20 epochs at
+    batch 256 (the screen's own training set-up), then scoring 8,192 rows
+    (one generation's screen).  FLOPs count the matrix products only, per
+    sample: 393,216 forward, 917,504 for a training step."""
+    import torch
+    from sigp.api import device_info
+    from sigp.surrogate import Mlp
+    rows, epochs = args.rules or 100_000, 20
+    rng = np.random.default_rng(1)
+    x = (rng.integers(0, 8, (rows, 512)) / 7.0).astype(np.float32)
+    w = rng.normal(size=512) / np.sqrt(512)
+    t = np.tanh(x @ w).astype(np.float32)
+    log("device", device_info()[0])
+    net = Mlp(seed=1)
+    net.train(x[:4096], t[:4096], epochs=1, batch=256)      # warm-up (code objects, buffers)
+    net.train(x, t, epochs=1, batch=256)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = net.train(x, t, epochs=epochs, batch=256, lr=1e-3, seed=3)
+    train_s = time.perf_counter() - t0
+    xs = x[:8192]
+    net.predict(xs)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        net.predict(xs)
+    pred_s = (time.perf_counter() - t0) / args.steps
+    samples = rows * epochs
+    achieved = MLP_TRAIN_FLOP * samples / train_s / 1e12
+    log("training %.3f s (%.1f us per step), losses %.4f -> %.4f; scoring 8,192 rows %.2f ms"
+        % (train_s, train_s / (samples / 256) * 1e6, losses[0], losses[-1], pred_s * 1e3))
+    out = {"metric": "surrogate training samples/sec", "value": round(samples / train_s, 1), "unit": "samples/s",
+           "n_gpus": 1, "steps": epochs, "warmup": 1, "ms_per_step": round(train_s / epochs * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+           "data": "synthetic phenotype-shaped rows",
+           "config": {"workload": "MLP 512-256-256-1, %d rows x %d epochs, batch 256, Adam lr 1e-3; then "
+                                  "scoring 8,192 rows" % (rows, epochs),
+                      "parallelism": "1 GPU; one wavefront per 16 x 16 output tile"},
+           "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": BF16_MFMA_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(achieved / BF16_MFMA_PEAK_TFLOPS, 6), "traffic": None,
+                        "flop_per_sample": MLP_TRAIN_FLOP, "note": "whole training wall time, host shuffle, "
+                        "upload and per-step launches included"},
+           "cpu_baseline": None, "train_s": round(train_s, 3), "score_8192_ms": round(pred_s * 1e3, 3),
+           "score_tflops": round(MLP_FWD_FLOP * 8192 / pred_s / 1e12, 4)}
+    print(json.dumps(out), flush=True)
+
+
 def bench_sampler(args):
     """Scenario rows on the device: the large configuration's 2,000 Gamma(2)
     rows of n = 288 (clustered + uniform, m = 4), each row its own
@@ -470,7 +523,8 @@ def main():
     ap.add_argument("--rules", type=int, default=None, help="rules (plain: 1,024; large: 4,096)")
     ap.add_argument("--scenarios", type=int, default=None, help="scenarios (plain: 500; large: 2,000)")
     ap.add_argument("--workload", choices=["plain", "gp-generation", "gp-reference", "batched", "large",
-                                           "gp-loop", "sampler", "timing", "ga-batch", "ga-search", "trace"],
+                                           "gp-loop", "sampler", "timing", "ga-batch", "ga-search", "trace",
+                                           "mlp"],
                     default="plain",
                     help="plain: the headline workload; gp-generation: one generation of the "
                          "on-device GP loop (1,024 rules x 50 scenarios, clustered n=100 m=3); "
@@ -484,7 +538,9 @@ def main():
                          "against its published training times; ga-batch: the GA's first batch "
                          "of 13 genomes x 16 seeds (--instance c101 has a logged time); ga-search: the "
                          "whole GA terminal-subset search (--instance c204 has a logged time); trace: "
-                         "gen_trace with all 18 extra terminals (--instance pr01 or c101 logged)")
+                         "gen_trace with all 18 extra terminals (--instance pr01 or c101 logged); mlp: "
+                         "the surrogate network's training (100,000 rows x 20 epochs) and scoring on "
+                         "the matrix cores")
     ap.add_argument("--generations", type=int, default=50, help="gp-loop: generations")
     ap.add_argument("--instance", default="c101", help="timing: instance (c101, r201, pr15 are published)")
     ap.add_argument("--terms", default="",
@@ -526,6 +582,8 @@ def main():
         return bench_ga_search(args)
     if args.workload == "trace":
         return bench_trace(args)
+    if args.workload == "mlp":
+        return bench_mlp(args)
     import torch
     from sigp.api import FactoryContext, batch_multi, device_info
     from sigp.workloads import (batched_uneven, gp_generation, gp_reference_generation, large,

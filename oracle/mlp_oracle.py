@@ -9,9 +9,19 @@ device's results differ only by its FP32 accumulation.  With emulate=False
 it is the plain float64 network, whose gradients are checked against
 central finite differences.
 
+adam_step() is the device's optimiser step (adam_kernel: beta1 0.9, beta2
+0.999, eps 1e-8, bias-corrected, Kingma & Ba's published update) in float32,
+in the kernel's operation order, with the host's powf for the bias
+corrections.
+
 Pins: finite differences (tests/test_mlp_oracle.py) for the gradient
-formulas; exact integer cases for the BF16 emulation.
+formulas; exact integer cases for the BF16 emulation; for adam_step, what
+the published update fixes: with zero moments the bias-corrected first step
+is lr * g / (|g| + eps), and under a constant gradient every later step is
+the same, which a wrong decay rate or a dropped correction breaks.
 """
+import ctypes
+import ctypes.util
 import numpy as np
 
 IN, H = 512, 256
@@ -76,3 +86,21 @@ def forward_backward(p, x, t, emulate=True):
 def loss_only(p, x, t, emulate=False):
     y, _ = forward(p, x, emulate)
     return float(np.mean((y - np.asarray(t, dtype=np.float64)) ** 2))
+
+
+_libm = ctypes.CDLL(ctypes.util.find_library("m"))
+_libm.powf.argtypes = [ctypes.c_float, ctypes.c_float]
+_libm.powf.restype = ctypes.c_float
+
+
+def adam_step(p, g, m, v, step, lr):
+    """One Adam step on float32 arrays as adam_kernel computes it; `step` is
+    the 1-based step count.  Returns (p, m, v) as float32."""
+    f = np.float32
+    p, g, m, v = (np.asarray(a, dtype=np.float32) for a in (p, g, m, v))
+    c1 = f(1.0) - f(_libm.powf(0.9, float(step)))
+    c2 = f(1.0) - f(_libm.powf(0.999, float(step)))
+    mk = f(0.9) * m + f(0.1) * g
+    vk = f(0.999) * v + f(0.001) * g * g
+    pk = p - f(lr) * (mk / c1) / (np.sqrt(vk / c2) + f(1e-8))
+    return pk.astype(np.float32), mk.astype(np.float32), vk.astype(np.float32)

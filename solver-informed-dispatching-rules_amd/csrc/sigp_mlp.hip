@@ -49,67 +49,90 @@ __host__ __device__ inline float from_bf16(uint16_t h) {
 enum Epi { EPI_STORE = 0, EPI_BIAS_RELU = 1, EPI_RELU_GRAD = 2 };
 
 // C[M x N] = A(M x K) . B(K x N) on BF16 operands (uint16 bits), FP32
-// accumulate.  A(i, k) = TA ? A[k * lda + i] : A[i * lda + k];
-// B(k, j) = TB ? B[j * ldb + k] : B[k * ldb + j].  One wavefront per 16 x 16
-// output tile, K in steps of 32 (M, N multiples of 16, K of 32).  MFMA
-// operand layout (16x16x32 bf16): lane l supplies row l % 16 of A and column
-// l % 16 of B at k = 8 * (l / 16) .. + 7, and receives rows 4 * (l / 16) .. + 3
-// of column l % 16 of the result.
+// accumulate, with both operands stored k-contiguous: A(i, k) = A[i * lda + k]
+// and B(k, j) = B[j * ldb + k] (B is held as B^T).  The producers of every
+// operand write it in that layout -- transposed BF16 copies of the weights
+// (Adam refreshes them) and of the activations and deltas that feed the
+// weight gradients (the epilogues write them) -- so each lane fetches its
+// MFMA operands as one 16-byte load per matrix per k-step, instead of 8
+// two-byte loads strided by a row.  One wavefront per 16 x 16 output tile,
+// K in steps of 32 (M, N multiples of 16, K of 32).  MFMA operand layout
+// (16x16x32 bf16): lane l supplies row l % 16 of A and column l % 16 of B at
+// k = 8 * (l / 16) .. + 7, and receives rows 4 * (l / 16) .. + 3 of column
+// l % 16 of the result.  The k-steps accumulate in order, as before the
+// layout change, so the results are the same bits.
 //   EPI_STORE:      out32 = C
 //   EPI_BIAS_RELU:  out32 = C + bias[j] (pre-activation), out16 = bf16(max(that, 0))
 //   EPI_RELU_GRAD:  out16 = bf16(C * (gate[i][j] > 0))   (gate: pre-activation, fp32)
-template <bool TA, bool TB, int EPI>
+// out16t (nullable) receives out16 transposed: out16t[j * ldt + i].
+template <int EPI>
 __global__ __launch_bounds__(256) void gemm_kernel(const uint16_t *__restrict__ A, int lda,
                                                    const uint16_t *__restrict__ B, int ldb, int M,
                                                    int N, int K, const float *__restrict__ bias,
                                                    const float *__restrict__ gate,
                                                    float *__restrict__ out32,
-                                                   uint16_t *__restrict__ out16) {
+                                                   uint16_t *__restrict__ out16,
+                                                   uint16_t *__restrict__ out16t, int ldt) {
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int tiles_n = N / 16;
     if (tile >= (M / 16) * tiles_n) return;
     const int i0 = (tile / tiles_n) * 16, j0 = (tile % tiles_n) * 16;
     const int r = lane & 15, kq = (lane >> 4) * 8;
+    const bf16x8 *ap = (const bf16x8 *)(A + (size_t)(i0 + r) * lda + kq);
+    const bf16x8 *bp = (const bf16x8 *)(B + (size_t)(j0 + r) * ldb + kq);
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 a, b;
+    int k0 = 0;
+    for (; k0 + 128 <= K; k0 += 128) {   // four k-steps' loads in flight together
+        bf16x8 a[4], b[4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int k = k0 + kq + t;
-            const uint16_t av = TA ? A[(size_t)k * lda + i0 + r] : A[(size_t)(i0 + r) * lda + k];
-            const uint16_t bv = TB ? B[(size_t)(j0 + r) * ldb + k] : B[(size_t)k * ldb + j0 + r];
-            a[t] = __builtin_bit_cast(__bf16, av);
-            b[t] = __builtin_bit_cast(__bf16, bv);
+        for (int u = 0; u < 4; ++u) {
+            a[u] = ap[k0 / 8 + 4 * u];
+            b[u] = bp[k0 / 8 + 4 * u];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], b[u], acc, 0, 0, 0);
     }
+    for (; k0 < K; k0 += 32) acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[k0 / 8], bp[k0 / 8], acc, 0, 0, 0);
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int i = i0 + (lane >> 4) * 4 + t, j = j0 + r;
         const size_t at = (size_t)i * N + j;
         float v = acc[t];
+        uint16_t h = 0;
         if (EPI == EPI_STORE) {
             out32[at] = v;
         } else if (EPI == EPI_BIAS_RELU) {
             v += bias[j];
             out32[at] = v;
-            out16[at] = to_bf16(v > 0.f ? v : 0.f);
+            h = to_bf16(v > 0.f ? v : 0.f);
+            out16[at] = h;
         } else {
-            out16[at] = to_bf16(gate[at] > 0.f ? v : 0.f);
+            h = to_bf16(gate[at] > 0.f ? v : 0.f);
+            out16[at] = h;
         }
+        if (EPI != EPI_STORE && out16t) out16t[(size_t)j * ldt + i] = h;
     }
+}
+
+// dst[c][r] = src[r][c] for a rows x cols BF16 matrix.
+__global__ void transpose16_kernel(const uint16_t *__restrict__ src, int rows, int cols,
+                                   uint16_t *__restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)rows * cols) return;
+    const int rr = (int)(k / cols), c = (int)(k - (int64_t)rr * cols);
+    dst[(size_t)c * rows + rr] = src[k];
 }
 
 // Output layer and its backward pass for rows b < nb of a batch padded to B
 // rows: y = A2 . w3 + b3 (FP32 dot in index order), MSE loss against t,
 // dy = 2 (y - t) / nb, delta2 = bf16(dy * w3 * (H2 > 0)); padded rows get
-// zero deltas.  One wavefront per row.
+// zero deltas; delta2t = delta2 transposed ([H][B]).  One wavefront per row.
 __global__ void head_kernel(const uint16_t *__restrict__ A2, const float *__restrict__ H2,
                             const float *__restrict__ w3, const float *__restrict__ b3,
                             const float *__restrict__ t, int nb, int B, float *__restrict__ y,
                             float *__restrict__ dy, float *__restrict__ sqerr,
-                            uint16_t *__restrict__ delta2, bool backward) {
+                            uint16_t *__restrict__ delta2, uint16_t *__restrict__ delta2t, bool backward) {
     const int lane = threadIdx.x & 63;
     const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (b >= B) return;
@@ -128,43 +151,65 @@ __global__ void head_kernel(const uint16_t *__restrict__ A2, const float *__rest
         }
         for (int j = lane; j < H; j += 64) {
             const size_t at = (size_t)b * H + j;
-            delta2[at] = to_bf16(H2[at] > 0.f ? g * w3[j] : 0.f);
+            const uint16_t h = to_bf16(H2[at] > 0.f ? g * w3[j] : 0.f);
+            delta2[at] = h;
+            delta2t[(size_t)j * B + b] = h;
         }
     }
 }
 
 // Gradients of the output layer and of the biases: gw3[j] = sum_b A2[b][j] dy[b],
-// gb3 = sum_b dy[b], gb2[j] = sum_b delta2[b][j], gb1[j] = sum_b delta1[b][j]
-// (each summed in row order by one thread).
-__global__ void small_grads_kernel(const uint16_t *__restrict__ A2, const float *__restrict__ dy,
-                                   const uint16_t *__restrict__ delta2,
-                                   const uint16_t *__restrict__ delta1, int B,
-                                   float *__restrict__ gw3, float *__restrict__ gb3,
-                                   float *__restrict__ gb2, float *__restrict__ gb1) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < H) {
-        float s3 = 0.f, s2 = 0.f, s1 = 0.f;
-        for (int b = 0; b < B; ++b) {
-            s3 += from_bf16(A2[(size_t)b * H + j]) * dy[b];
-            s2 += from_bf16(delta2[(size_t)b * H + j]);
-            s1 += from_bf16(delta1[(size_t)b * H + j]);
-        }
-        gw3[j] = s3;
-        gb2[j] = s2;
-        gb1[j] = s1;
+// gb3 = sum_b dy[b], gb2[j] = sum_b delta2[b][j], gb1[j] = sum_b delta1[b][j].
+// Workgroup = 16 columns j; its 256 threads split the B rows into 16 chunks
+// (thread t: column t % 16, chunk t / 16), each summed in row order, and the
+// 16 partial sums of a column are then added in chunk order -- a fixed order,
+// so the result is deterministic.  (One thread per column summing all B rows
+// serially took 71 us per training step, more than the five matrix products
+// together.)
+__global__ __launch_bounds__(256) void small_grads_kernel(const uint16_t *__restrict__ A2,
+                                                          const float *__restrict__ dy,
+                                                          const uint16_t *__restrict__ delta2,
+                                                          const uint16_t *__restrict__ delta1, int B,
+                                                          float *__restrict__ gw3, float *__restrict__ gb3,
+                                                          float *__restrict__ gb2, float *__restrict__ gb1) {
+    __shared__ float part[4][16][16];
+    const int jj = threadIdx.x & 15, c = threadIdx.x >> 4;
+    const int j = blockIdx.x * 16 + jj;
+    const int rows = B / 16, b0 = c * rows;
+    float s3 = 0.f, s2 = 0.f, s1 = 0.f, sd = 0.f;
+    for (int b = b0; b < b0 + rows; ++b) {
+        const float d = dy[b];
+        s3 += from_bf16(A2[(size_t)b * H + j]) * d;
+        s2 += from_bf16(delta2[(size_t)b * H + j]);
+        s1 += from_bf16(delta1[(size_t)b * H + j]);
+        sd += d;
     }
-    if (j == 0) {
-        float s = 0.f;
-        for (int b = 0; b < B; ++b) s += dy[b];
-        gb3[0] = s;
+    part[0][c][jj] = s3;
+    part[1][c][jj] = s2;
+    part[2][c][jj] = s1;
+    part[3][c][jj] = sd;
+    __syncthreads();
+    if (c == 0) {
+        float t3 = 0.f, t2 = 0.f, t1 = 0.f, td = 0.f;
+        for (int q = 0; q < 16; ++q) {
+            t3 += part[0][q][jj];
+            t2 += part[1][q][jj];
+            t1 += part[2][q][jj];
+            td += part[3][q][jj];
+        }
+        gw3[j] = t3;
+        gb2[j] = t2;
+        gb1[j] = t1;
+        if (blockIdx.x == 0 && jj == 0) gb3[0] = td;
     }
 }
 
 // Adam (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected) on the FP32
-// parameters; refreshes the BF16 copies of W1 and W2.
+// parameters; refreshes the BF16 copies of W1 and W2 and their transposes.
 __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
                             float *__restrict__ v, int n, float lr, float c1, float c2,
-                            uint16_t *__restrict__ w1_16, uint16_t *__restrict__ w2_16) {
+                            uint16_t *__restrict__ w1_16, uint16_t *__restrict__ w2_16,
+                            uint16_t *__restrict__ w1t_16, uint16_t *__restrict__ w2t_16) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const float gk = g[k];
@@ -174,20 +219,48 @@ __global__ void adam_kernel(float *__restrict__ p, const float *__restrict__ g, 
     v[k] = vk;
     const float pk = p[k] - lr * (mk / c1) / (sqrtf(vk / c2) + 1e-8f);
     p[k] = pk;
-    if (k < IN * H) w1_16[k] = to_bf16(pk);
-    else if (k < IN * H + H + H * H && k >= IN * H + H) w2_16[k - IN * H - H] = to_bf16(pk);
+    if (k < IN * H) {
+        const uint16_t h = to_bf16(pk);
+        w1_16[k] = h;
+        w1t_16[(size_t)(k % H) * IN + k / H] = h;
+    } else if (k < IN * H + H + H * H && k >= IN * H + H) {
+        const int q = k - IN * H - H;
+        const uint16_t h = to_bf16(pk);
+        w2_16[q] = h;
+        w2t_16[(size_t)(q % H) * H + q / H] = h;
+    }
 }
 
+// Training batch rows b < nb from rows idx[b] of the resident set, zero rows
+// up to B: xb [B][IN], its transpose xt [IN][B] (the W1 gradient's operand)
+// and the targets tb.
 __global__ void gather_kernel(const uint16_t *__restrict__ X, const float *__restrict__ T,
                               const int64_t *__restrict__ idx, int nb, int B,
-                              uint16_t *__restrict__ xb, float *__restrict__ tb) {
+                              uint16_t *__restrict__ xb, uint16_t *__restrict__ xt, float *__restrict__ tb) {
     const int b = blockIdx.x;
     if (b >= B) return;
     const bool live = b < nb;
     const int64_t src = live ? idx[b] : 0;
-    for (int f = threadIdx.x; f < IN; f += blockDim.x)
-        xb[(size_t)b * IN + f] = live ? X[(size_t)src * IN + f] : (uint16_t)0;
+    for (int f = threadIdx.x; f < IN; f += blockDim.x) {
+        const uint16_t h = live ? X[(size_t)src * IN + f] : (uint16_t)0;
+        xb[(size_t)b * IN + f] = h;
+        xt[(size_t)f * B + b] = h;
+    }
     if (threadIdx.x == 0) tb[b] = live ? T[src] : 0.f;
+}
+
+// Sum of the batch's squared errors into the epoch's loss, in row order, in
+// double (the order and type the host summed in); the rows are staged in
+// LDS first so that the serial sum does not wait on one global load per row.
+__global__ __launch_bounds__(256) void loss_acc_kernel(const float *__restrict__ se, int nb,
+                                                       double *__restrict__ acc) {
+    __shared__ float sv[MAXB];
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) sv[b] = se[b];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double s = *acc;
+    for (int b = 0; b < nb; ++b) s += sv[b];
+    *acc = s;
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ x, int64_t n, uint16_t *__restrict__ out) {
@@ -205,19 +278,22 @@ static_assert(NP == SIGP_MLP_NPARAMS, "parameter count");
 
 struct sigp_mlp {
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;   // NP each
-    uint16_t *w1 = nullptr, *w2 = nullptr;                          // BF16 copies
-    uint16_t *xb = nullptr, *a1 = nullptr, *a2 = nullptr, *d1 = nullptr, *d2 = nullptr;
+    uint16_t *w1 = nullptr, *w2 = nullptr;                          // BF16 copies [in][out]
+    uint16_t *w1t = nullptr, *w2t = nullptr;                        // and transposed [out][in]
+    uint16_t *xb = nullptr, *a1 = nullptr, *a2 = nullptr, *d1 = nullptr, *d2 = nullptr;   // [B][.]
+    uint16_t *xt = nullptr, *a1t = nullptr, *d1t = nullptr, *d2t = nullptr;              // [.][B]
     float *h1 = nullptr, *h2 = nullptr, *tb = nullptr, *y = nullptr, *dy = nullptr, *se = nullptr;
-    int64_t *idx = nullptr;
     uint16_t *data = nullptr;   // resident training inputs (BF16)
     float *targets = nullptr;
+    int64_t *order = nullptr;   // the epoch's shuffled row order
     int64_t data_cap = 0;
     long long steps = 0;        // Adam step count
 };
 
 static void free_mlp(sigp_mlp *n) {
-    void *ptrs[] = {n->p, n->g, n->m, n->v, n->w1, n->w2, n->xb, n->a1, n->a2, n->d1, n->d2,
-                    n->h1, n->h2, n->tb, n->y, n->dy, n->se, n->idx, n->data, n->targets};
+    void *ptrs[] = {n->p, n->g, n->m, n->v, n->w1, n->w2, n->w1t, n->w2t, n->xb, n->a1, n->a2, n->d1,
+                    n->d2, n->xt, n->a1t, n->d1t, n->d2t, n->h1, n->h2, n->tb, n->y, n->dy, n->se,
+                    n->data, n->targets, n->order};
     for (void *q : ptrs)
         if (q) (void)hipFree(q);
     delete n;
@@ -247,6 +323,8 @@ static void init_params(uint64_t seed, std::vector<float> &p) {
 static sigp_status refresh_bf16(sigp_mlp *n) {
     to_bf16_kernel<<<(IN * H + 255) / 256, 256>>>(n->p + O_W1, IN * H, n->w1);
     to_bf16_kernel<<<(H * H + 255) / 256, 256>>>(n->p + O_W2, H * H, n->w2);
+    transpose16_kernel<<<(IN * H + 255) / 256, 256>>>(n->w1, IN, H, n->w1t);
+    transpose16_kernel<<<(H * H + 255) / 256, 256>>>(n->w2, H, H, n->w2t);
     MLP_TRY(hipGetLastError());
     return SIGP_OK;
 }
@@ -258,13 +336,15 @@ extern "C" sigp_status sigp_mlp_create(uint64_t seed, sigp_mlp **out) {
     auto alloc = [&](void **q, size_t bytes) { return hipMalloc(q, bytes) == hipSuccess; };
     bool ok = alloc((void **)&n->p, NP * 4) && alloc((void **)&n->g, NP * 4) && alloc((void **)&n->m, NP * 4) &&
               alloc((void **)&n->v, NP * 4) && alloc((void **)&n->w1, IN * H * 2) &&
-              alloc((void **)&n->w2, H * H * 2) && alloc((void **)&n->xb, (size_t)MAXB * IN * 2) &&
+              alloc((void **)&n->w2, H * H * 2) && alloc((void **)&n->w1t, IN * H * 2) &&
+              alloc((void **)&n->w2t, H * H * 2) && alloc((void **)&n->xt, (size_t)MAXB * IN * 2) &&
+              alloc((void **)&n->a1t, (size_t)MAXB * H * 2) && alloc((void **)&n->d1t, (size_t)MAXB * H * 2) &&
+              alloc((void **)&n->d2t, (size_t)MAXB * H * 2) && alloc((void **)&n->xb, (size_t)MAXB * IN * 2) &&
               alloc((void **)&n->a1, (size_t)MAXB * H * 2) && alloc((void **)&n->a2, (size_t)MAXB * H * 2) &&
               alloc((void **)&n->d1, (size_t)MAXB * H * 2) && alloc((void **)&n->d2, (size_t)MAXB * H * 2) &&
               alloc((void **)&n->h1, (size_t)MAXB * H * 4) && alloc((void **)&n->h2, (size_t)MAXB * H * 4) &&
               alloc((void **)&n->tb, MAXB * 4) && alloc((void **)&n->y, MAXB * 4) &&
-              alloc((void **)&n->dy, MAXB * 4) && alloc((void **)&n->se, MAXB * 4) &&
-              alloc((void **)&n->idx, MAXB * 8);
+              alloc((void **)&n->dy, MAXB * 4) && alloc((void **)&n->se, MAXB * 4);
     if (!ok) { free_mlp(n); return sigp_internal_fail(SIGP_OOM, "MLP allocation failed"); }
     std::vector<float> p;
     init_params(seed, p);
@@ -299,27 +379,31 @@ extern "C" sigp_status sigp_mlp_set_params(sigp_mlp *n, const float *in) {
 }
 
 // Forward pass (and, with targets, backward pass into n->g) of the batch
-// already in n->xb: nb live rows padded to B (a multiple of 16).
+// already in n->xb (and, for the backward pass, n->xt): nb live rows padded
+// to B (a multiple of 32).  Every product takes its operands k-contiguous
+// (gemm_kernel): the forward pass reads the transposed weights, the
+// backward pass W2 itself and the transposed activations and deltas the
+// forward epilogues and the head wrote.
 static sigp_status run_batch(sigp_mlp *n, int nb, int B, bool backward, hipStream_t stream = nullptr) {
     const int tiles_h = (B / 16) * (H / 16);
     const int grid_h = (tiles_h + 3) / 4;
-    gemm_kernel<false, false, EPI_BIAS_RELU><<<grid_h, 256, 0, stream>>>(n->xb, IN, n->w1, H, B, H, IN, n->p + O_B1,
-                                                              nullptr, n->h1, n->a1);
-    gemm_kernel<false, false, EPI_BIAS_RELU><<<grid_h, 256, 0, stream>>>(n->a1, H, n->w2, H, B, H, H, n->p + O_B2,
-                                                              nullptr, n->h2, n->a2);
+    gemm_kernel<EPI_BIAS_RELU><<<grid_h, 256, 0, stream>>>(n->xb, IN, n->w1t, IN, B, H, IN, n->p + O_B1,
+                                                           nullptr, n->h1, n->a1, backward ? n->a1t : nullptr, B);
+    gemm_kernel<EPI_BIAS_RELU><<<grid_h, 256, 0, stream>>>(n->a1, H, n->w2t, H, B, H, H, n->p + O_B2,
+                                                           nullptr, n->h2, n->a2, nullptr, B);
     head_kernel<<<(B + 3) / 4, 256, 0, stream>>>(n->a2, n->h2, n->p + O_W3, n->p + O_B3, n->tb, nb, B, n->y, n->dy,
-                                      n->se, n->d2, backward);
+                                                 n->se, n->d2, n->d2t, backward);
     if (backward) {
-        // delta1 = (delta2 . W2^T) * (H1 > 0)
-        gemm_kernel<false, true, EPI_RELU_GRAD><<<grid_h, 256, 0, stream>>>(n->d2, H, n->w2, H, B, H, H, nullptr,
-                                                                 n->h1, nullptr, n->d1);
-        // dW2 = A1^T . delta2, dW1 = X^T . delta1
-        gemm_kernel<true, false, EPI_STORE><<<((H / 16) * (H / 16) + 3) / 4, 256, 0, stream>>>(
-            n->a1, H, n->d2, H, H, H, B, nullptr, nullptr, n->g + O_W2, nullptr);
-        gemm_kernel<true, false, EPI_STORE><<<((IN / 16) * (H / 16) + 3) / 4, 256, 0, stream>>>(
-            n->xb, IN, n->d1, H, IN, H, B, nullptr, nullptr, n->g + O_W1, nullptr);
-        small_grads_kernel<<<1, 256, 0, stream>>>(n->a2, n->dy, n->d2, n->d1, B, n->g + O_W3, n->g + O_B3,
-                                       n->g + O_B2, n->g + O_B1);
+        // delta1 = (delta2 . W2^T) * (H1 > 0): B(k, j) = W2[j][k]
+        gemm_kernel<EPI_RELU_GRAD><<<grid_h, 256, 0, stream>>>(n->d2, H, n->w2, H, B, H, H, nullptr, n->h1,
+                                                               nullptr, n->d1, n->d1t, B);
+        // dW2 = A1^T . delta2, dW1 = X^T . delta1 (K = B)
+        gemm_kernel<EPI_STORE><<<((H / 16) * (H / 16) + 3) / 4, 256, 0, stream>>>(
+            n->a1t, B, n->d2t, B, H, H, B, nullptr, nullptr, n->g + O_W2, nullptr, nullptr, 0);
+        gemm_kernel<EPI_STORE><<<((IN / 16) * (H / 16) + 3) / 4, 256, 0, stream>>>(
+            n->xt, B, n->d1t, B, IN, H, B, nullptr, nullptr, n->g + O_W1, nullptr, nullptr, 0);
+        small_grads_kernel<<<H / 16, 256, 0, stream>>>(n->a2, n->dy, n->d2, n->d1, B, n->g + O_W3,
+                                                       n->g + O_B3, n->g + O_B2, n->g + O_B1);
     }
     MLP_TRY(hipGetLastError());
     return SIGP_OK;
@@ -379,7 +463,9 @@ extern "C" sigp_status sigp_mlp_gradients(sigp_mlp *n, const float *x, const flo
     if (nb < 1 || nb > MAXB) return sigp_internal_fail(SIGP_INVALID_ARG, "batch must hold 1..1024 rows");
     sigp_status st = load_batch(n, x, t, nb);
     if (st != SIGP_OK) return st;
-    if ((st = run_batch(n, nb, padded(nb), true)) != SIGP_OK) return st;
+    const int B = padded(nb);
+    transpose16_kernel<<<(B * IN + 255) / 256, 256>>>(n->xb, B, IN, n->xt);
+    if ((st = run_batch(n, nb, B, true)) != SIGP_OK) return st;
     MLP_TRY(hipMemcpy(grad_out, n->g, NP * 4, hipMemcpyDeviceToHost));
     if (loss_out) {
         std::vector<float> se(nb);
@@ -413,11 +499,14 @@ extern "C" sigp_status sigp_mlp_train(sigp_mlp *n, const float *x, const float *
     if (n->data_cap < rows) {
         if (n->data) (void)hipFree(n->data);
         if (n->targets) (void)hipFree(n->targets);
+        if (n->order) (void)hipFree(n->order);
         n->data = nullptr;
         n->targets = nullptr;
+        n->order = nullptr;
         n->data_cap = 0;
         MLP_TRY(hipMalloc((void **)&n->data, (size_t)rows * IN * 2));
         MLP_TRY(hipMalloc((void **)&n->targets, (size_t)rows * 4));
+        MLP_TRY(hipMalloc((void **)&n->order, (size_t)rows * 8));
         n->data_cap = rows;
     }
     {
@@ -436,28 +525,44 @@ extern "C" sigp_status sigp_mlp_train(sigp_mlp *n, const float *x, const float *
         z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
         return z ^ (z >> 31);
     };
-    std::vector<float> se(MAXB);
-    for (int e = 0; e < epochs; ++e) {
+    // Per epoch the shuffled order goes up once; each step then runs on the
+    // device with no host round trip (the batch rows come from the order,
+    // the squared errors add up on the device in the host's order and type).
+    double *acc = nullptr;
+    MLP_TRY(hipMalloc((void **)&acc, sizeof(double) * (size_t)std::max(epochs, 1)));
+    if (hipMemset(acc, 0, sizeof(double) * (size_t)std::max(epochs, 1)) != hipSuccess) {
+        (void)hipFree(acc);
+        return sigp_internal_fail(SIGP_DEVICE_ERROR, "hipMemset failed");
+    }
+    sigp_status st = SIGP_OK;
+    for (int e = 0; e < epochs && st == SIGP_OK; ++e) {
         for (int64_t k = rows - 1; k > 0; --k) std::swap(order[k], order[next() % (uint64_t)(k + 1)]);
-        double loss = 0.0;
-        for (int64_t r0 = 0; r0 < rows; r0 += batch) {
+        if (hipMemcpy(n->order, order.data(), (size_t)rows * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            st = sigp_internal_fail(SIGP_DEVICE_ERROR, "order upload failed");
+            break;
+        }
+        for (int64_t r0 = 0; r0 < rows && st == SIGP_OK; r0 += batch) {
             const int nb = (int)std::min<int64_t>(batch, rows - r0);
             const int B = padded(nb);
-            MLP_TRY(hipMemcpy(n->idx, order.data() + r0, (size_t)nb * 8, hipMemcpyHostToDevice));
-            gather_kernel<<<B, 256>>>(n->data, n->targets, n->idx, nb, B, n->xb, n->tb);
-            sigp_status st = run_batch(n, nb, B, true);
-            if (st != SIGP_OK) return st;
+            gather_kernel<<<B, 256>>>(n->data, n->targets, n->order + r0, nb, B, n->xb, n->xt, n->tb);
+            st = run_batch(n, nb, B, true);
+            if (st != SIGP_OK) break;
             ++n->steps;
             const float c1 = 1.f - std::pow(0.9f, (float)n->steps), c2 = 1.f - std::pow(0.999f, (float)n->steps);
-            adam_kernel<<<(NP + 255) / 256, 256>>>(n->p, n->g, n->m, n->v, NP, lr, c1, c2, n->w1, n->w2);
-            MLP_TRY(hipGetLastError());
-            if (epoch_loss) {
-                MLP_TRY(hipMemcpy(se.data(), n->se, (size_t)nb * 4, hipMemcpyDeviceToHost));
-                for (int b = 0; b < nb; ++b) loss += se[b];
-            }
+            adam_kernel<<<(NP + 255) / 256, 256>>>(n->p, n->g, n->m, n->v, NP, lr, c1, c2, n->w1, n->w2, n->w1t,
+                                                   n->w2t);
+            if (epoch_loss) loss_acc_kernel<<<1, 256>>>(n->se, nb, acc + e);
+            if (hipGetLastError() != hipSuccess) st = sigp_internal_fail(SIGP_DEVICE_ERROR, "training launch failed");
         }
-        if (epoch_loss) epoch_loss[e] = (float)(loss / rows);
     }
+    if (st == SIGP_OK && epoch_loss && epochs > 0) {
+        std::vector<double> sums(epochs);
+        if (hipMemcpy(sums.data(), acc, sizeof(double) * epochs, hipMemcpyDeviceToHost) != hipSuccess)
+            st = sigp_internal_fail(SIGP_DEVICE_ERROR, "loss download failed");
+        for (int e = 0; e < epochs && st == SIGP_OK; ++e) epoch_loss[e] = (float)(sums[e] / rows);
+    }
+    (void)hipFree(acc);
+    if (st != SIGP_OK) return st;
     MLP_TRY(hipDeviceSynchronize());
     return SIGP_OK;
 }

@@ -63,6 +63,34 @@ def test_gradients_match_oracle():
         assert err < 2e-3, name
 
 
+def test_adam_steps_match_oracle():
+    """Two training steps on one row (batch 1, so each epoch is one step and
+    the row order is fixed) equal the oracle's Adam (float32, kernel order)
+    applied to the device's own gradients, bit for bit: pins adam_kernel,
+    its bias corrections and the refresh of the BF16 weight copies the
+    second gradient is taken with."""
+    from sigp.surrogate import Mlp
+    rng = np.random.default_rng(21)
+    x = (rng.integers(0, 8, (1, mo.IN)) / 7.0).astype(np.float32)
+    t = np.array([0.7], dtype=np.float32)
+    lr = np.float32(1e-3)
+    net = Mlp(seed=5)
+    p0 = net.params()
+    g0, _ = net.gradients(x, t)
+    net.train(x, t, epochs=2, batch=1, lr=float(lr), seed=0)
+    got = net.params()
+    z = np.zeros_like(p0)
+    p1, m1, v1 = mo.adam_step(p0, g0, z, z, 1, lr)
+    probe = Mlp(seed=6)
+    probe.set_params(p1)
+    g1, _ = probe.gradients(x, t)
+    p2, _, _ = mo.adam_step(p1, g1, m1, v1, 2, lr)
+    moved = np.count_nonzero(p2 != p0)
+    print("parameters moved by two steps: %d of %d" % (moved, p0.size), flush=True)
+    assert moved > p0.size // 4
+    assert np.array_equal(got.view(np.uint32), p2.view(np.uint32))
+
+
 def test_training_fits_a_known_function():
     from sigp.surrogate import Mlp, spearman
     rng = np.random.default_rng(9)

@@ -1,6 +1,7 @@
 """The surrogate network's CPU oracle: its float64 gradients match central
-finite differences, and its BF16 rounding matches the definition
-(round to nearest even on the float32 bits)."""
+finite differences, its BF16 rounding matches the definition (round to
+nearest even on the float32 bits), and its Adam step gives what the
+published update fixes."""
 import numpy as np
 
 import mlp_oracle as mo
@@ -36,3 +37,26 @@ def test_bf16_rounding():
     assert got[2] == 1.0 + 2 ** -7
     assert got[4] == -2.5 and got[5] == 0.0
     assert abs(got[6] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+def test_adam_step_closed_forms():
+    """With zero moments the first bias-corrected step is lr * g / (|g| + eps);
+    under a constant gradient the bias-corrected moments stay g and g^2, so
+    every step moves by the same amount.  A wrong beta, a dropped bias
+    correction or a transposed m / v each break one of these."""
+    rng = np.random.default_rng(8)
+    g = (rng.normal(size=4000) * 10.0 ** rng.uniform(-4, 1, 4000)).astype(np.float32)
+    p0 = rng.normal(size=4000).astype(np.float32)
+    lr = np.float32(1e-3)
+    z = np.zeros_like(g)
+    p1, m1, v1 = mo.adam_step(p0, g, z, z, 1, lr)
+    want = lr * g.astype(np.float64) / (np.abs(g.astype(np.float64)) + 1e-8)
+    assert np.allclose(p0.astype(np.float64) - p1, want, rtol=2e-3, atol=2e-7)
+    p, m, v = p1, m1, v1
+    for step in range(2, 12):
+        pn, m, v = mo.adam_step(p, g, m, v, step, lr)
+        assert np.allclose(p.astype(np.float64) - pn, want, rtol=2e-3, atol=2e-7), step
+        p = pn
+    # the moments are the uncorrected exponential averages
+    assert np.allclose(m, (1 - 0.9 ** 11) * g, rtol=1e-5)
+    assert np.allclose(v, (1 - 0.999 ** 11) * g.astype(np.float64) ** 2, rtol=1e-4)
