@@ -230,7 +230,9 @@ __device__ __forceinline__ double scan_nstw(const FusedTabs &F, int n, int c, do
 // time: the body holds two erf evaluations.  A lane whose difference lies
 // beyond its divisor's saturation threshold (DevInst::atp*, atq*) takes the
 // value norm_cdf would return there, exactly 1 or 0, without the division
-// and the erf; the erf path runs only for lanes that need it.
+// and the erf; the erf path runs only for lanes that need it.  A lane whose
+// mean arrival at j is not after open_j starts j at open_j with no
+// variance, so its p_bud is j's own constant (DevInst::atpb0).
 template <int NW>
 __device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &F, int c, double fin_c,
                              const uint32_t (&vis)[NW], const uint32_t (&cnow)[NW]) {
@@ -251,34 +253,43 @@ __device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &
         if (!act) continue;
         const double ma = fin_c + dd[j];
         const double cl = F.close[j];
-        double pw;
+        // Each factor is first classified without erf: saturated (exactly 0
+        // or 1: DevInst::atp*, atq*, atpb0) or needing it.  1 - pw * pb is 1
+        // whenever either factor is 0 (both lie in [0, 1]), so a factor's
+        // erf is skipped when the other is exactly 0.
+        double pw, dcs = 0.0;
+        bool pw_e = false;
         if (sdc > 1e-9) {
-            const double dc = cl - ma;
-            pw = dc >= p_hi ? 1.0 : 0.0;
-            if (dc < p_hi && dc > p_lo) pw = norm_cdf(dc / sdc);
+            dcs = cl - ma;
+            pw = dcs >= p_hi ? 1.0 : 0.0;
+            pw_e = dcs < p_hi && dcs > p_lo;
         } else {
             pw = ma <= cl ? 1.0 : 0.0;
         }
-        double pb = 1.0;
-        if (pw != 0.0) {   // 1 - 0 * pb == 1 for every pb in [0, 1]
+        double pb = 1.0, es = 0.0, pdiv = 1.0;
+        bool pb_e = false;
+        if (pw_e || pw != 0.0) {
             const double op = F.open[j];
-            const bool late = ma > op;
-            const double ms = late ? ma : op;
-            const double vs = late ? sdc2 : 0.0;
-            const double sdj = F.sdv[j];
-            const double vb = vs + sdj * sdj;
-            const double mbd = ms + F.dur[j] + F.d2d[j];
-            if (vb > 1e-18) {
-                const int kj = I.dcls[j];
-                const double e = I.tmax - mbd;
-                const double q_hi = late ? qrow[2 * kj] : I.atq1[2 * kj];
-                const double q_lo = late ? qrow[2 * kj + 1] : I.atq1[2 * kj + 1];
-                pb = e >= q_hi ? 1.0 : 0.0;
-                if (e < q_hi && e > q_lo) pb = norm_cdf(e / (late ? pwrow[kj] : I.pw1[kj]));
+            if (ma > op) {   // late: the start time inherits the candidate's variance
+                const double sdj = F.sdv[j];
+                const double vb = sdc2 + sdj * sdj;
+                const double mbd = ma + F.dur[j] + F.d2d[j];
+                if (vb > 1e-18) {
+                    const int kj = I.dcls[j];
+                    es = I.tmax - mbd;
+                    const double q_hi = qrow[2 * kj], q_lo = qrow[2 * kj + 1];
+                    pb = es >= q_hi ? 1.0 : 0.0;
+                    pb_e = es < q_hi && es > q_lo;
+                    pdiv = pwrow[kj];
+                } else {
+                    pb = mbd <= I.tmax ? 1.0 : 0.0;
+                }
             } else {
-                pb = mbd <= I.tmax ? 1.0 : 0.0;
+                pb = I.atpb0[j];   // starts at open_j: the per-customer value
             }
         }
+        if (pw_e && (pb_e || pb != 0.0)) pw = norm_cdf(dcs / sdc);
+        if (pb_e && pw != 0.0) pb = norm_cdf(es / pdiv);
         el += F.score[j] * (1.0 - pw * pb);
     }
     return el;

@@ -1257,6 +1257,17 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
             atq1[2 * a + 1] = ncdf_zero_to(pw1[a]);
         }
     }
+    // ATRISK's p_bud when the mean arrival at j is not after open_j
+    // (engine/sim_terms.hpp:243-252: mean_start = open_j, var_start = 0.0),
+    // in the reference's expression with the host's erf
+    std::vector<double> atpb0(n, 1.0);
+    for (int j = 0; j < n; ++j) {
+        const double sdj = SIG * d->durations[j];
+        const double vb = 0.0 + sdj * sdj;
+        const double mbd = d->open_t[j] + d->durations[j] + d->dist_to_depot[j];
+        if (vb > 1e-18) atpb0[j] = 0.5 * (1.0 + std::erf(((d->tmax - mbd) / pw1[cls[j]]) * 0.7071067811865476));
+        else atpb0[j] = mbd <= d->tmax ? 1.0 : 0.0;
+    }
 
     const size_t nn = (size_t)ld * ld;
     const size_t np = (size_t)std::max(8, (n + 7) / 8 * 8);   // customer arrays padded for 4-wide reads
@@ -1268,7 +1279,7 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
                  o_score = take(np * 8), o_durrob = take(np * 8), o_sdv = take(np * 8),
                  o_cls = take(n * 4 + 8), o_pw2 = take((size_t)K * K * 8), o_pw1 = take(K * 8),
                  o_atp1 = take(K * 8), o_atp0 = take(K * 8), o_atq2 = take((size_t)K * K * 16),
-                 o_atq1 = take((size_t)K * 16),
+                 o_atq1 = take((size_t)K * 16), o_atpb0 = take(np * 8),
                  o_zp = take(10 * 8), o_feat = take((size_t)n * 5 * 8 + 8), o_ok = take(n * 4 + 8);
     char *base = nullptr;
     HIP_TRY(hipMalloc((void **)&base, off));
@@ -1292,6 +1303,7 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     HIP_TRY(up(o_atp0, atp0.data(), K * 8));
     HIP_TRY(up(o_atq2, atq2.data(), (size_t)K * K * 16));
     HIP_TRY(up(o_atq1, atq1.data(), (size_t)K * 16));
+    HIP_TRY(up(o_atpb0, atpb0.data(), (size_t)n * 8));
 
     DevInst &D = I->dev;
     D.n = n;
@@ -1318,6 +1330,7 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     D.atp0 = cd(o_atp0);
     D.atq2 = cd(o_atq2);
     D.atq1 = cd(o_atq1);
+    D.atpb0 = cd(o_atpb0);
     D.zp = cd(o_zp);
 
     {

@@ -51,6 +51,11 @@ struct DevInst {
     // lo) and per pw1 entry (atq1: K pairs hi, lo).
     cdp atp1, atp0;
     cdp atq2, atq1;
+    // ATRISK's p_bud of customer j for a mean arrival before j opens
+    // (engine/sim_terms.hpp:243-252 with mean_start = open_j, var_start = 0):
+    // it depends on j alone, so the host computes it once per instance with
+    // the C library's erf -- the reference's own value.
+    cdp atpb0;
     cdp zp;      // 10 z-statistics
 };
 
