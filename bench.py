@@ -560,6 +560,9 @@ def main():
                     help="plain / large / batched: vehicles = sim_one (the reference's V0, the "
                          "headline); rr = sim_one_rr, the round-robin ablation arm (no roofline: the "
                          "committed work models count V0)")
+    ap.add_argument("--one-launch", action="store_true",
+                    help="batched: every size class in one launch (sigp_instance_one_launch) instead of one "
+                         "launch per class on concurrent streams")
     ap.add_argument("--no-scenario-order", action="store_true",
                     help="scenario lanes in index order (sigp_batch_order_scenarios off)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -610,6 +613,7 @@ def main():
         ctxs = [FactoryContext(i) for i in wl.insts]
         for c in ctxs:
             c.protocol(rr)
+            c.one_launch(args.one_launch)
         ctx = ctxs[0]
         batch = batch_multi(ctxs, wl.packed, T, wl.mask)
         n_rows = len(wl.insts) * T
@@ -706,7 +710,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": describe(args.workload, wl) + ("; round-robin protocol (sim_one_rr)" if rr else ""),
+        "config": {"workload": describe(args.workload, wl) + ("; round-robin protocol (sim_one_rr)" if rr else "")
+                   + ("; every size class in one launch" if args.one_launch and args.workload == "batched" else ""),
                    "rules": P, "scenarios": T,
                    "n": wl.inst.n if hasattr(wl, "inst") else None,
                    "m": wl.inst.m if hasattr(wl, "inst") else None,

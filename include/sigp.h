@@ -238,6 +238,17 @@ sigp_status sigp_instance_skip_repeats(sigp_instance *inst, int32_t enable);
  * ones sigp_evaluate creates internally. */
 sigp_status sigp_instance_configure(sigp_instance *inst, const sigp_launch_config *cfg);
 
+/* One launch for a multi-instance batch (no counterpart in the reference,
+ * which launches per instance: engine/factory.cuh:42-112 behind each
+ * FactoryContext).  When every instance of a batch created later by
+ * sigp_batch_create_multi / sigp_evaluate_multi has this enabled, all
+ * (instance, rule) pairs run in one launch of the widest size class's
+ * kernels, in one longest-first order, instead of one launch per size class
+ * on concurrent streams.  Results are identical either way; off by default
+ * (the per-class launches measured as fast, and the narrower classes' kernels
+ * hold fewer registers). */
+sigp_status sigp_instance_one_launch(sigp_instance *inst, int32_t enable);
+
 /* Simulation protocol.  SIGP_PROTOCOL_VEHICLES is sim_one
  * (engine/sim_core.hpp:207-238, the reference's V0 and its only GPU path):
  * vehicle 0 routes until pick_best finds no candidate, then vehicle 1, ...

@@ -1056,6 +1056,7 @@ struct sigp_instance {
     int mapping = SIGP_MAP_AUTO;
     bool skip_repeats = false;      // new batches simulate each distinct rule once
     int protocol = SIGP_PROTOCOL_VEHICLES;   // protocol of new batches
+    bool one_launch = false;        // multi-instance batches: every size class in one launch
 };
 
 // Launch order and tuned mapping of the instances of one size class (one
@@ -1597,7 +1598,6 @@ struct BatchPlan {
     long long perms = 0;
 };
 
-// Fills bt's host fields (shape, size classes, variant flags) and the plan.
 // The round-robin protocol keeps 2 * (largest m) LDS rows per wavefront.
 static sigp_status set_protocol(sigp_batch *bt, int protocol) {
     if (protocol != SIGP_PROTOCOL_VEHICLES && protocol != SIGP_PROTOCOL_ROUND_ROBIN)
@@ -1617,6 +1617,7 @@ static sigp_status set_protocol(sigp_batch *bt, int protocol) {
     return SIGP_OK;
 }
 
+// Fills bt's host fields (shape, size classes, variant flags) and the plan.
 static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool multi, const PreparedRules &pr,
                               int T, int group_size, sigp_batch *bt, BatchPlan &pl) {
     const int P = pr.P, K = (int)insts.size();
@@ -1652,6 +1653,16 @@ static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool mu
     }
     pl.perms = perms;
     const int classes[3] = {4, 9, 16};
+    // One launch over every instance when all of them ask for it: every
+    // instance then runs in the widest class's kernels (a narrower visited
+    // set is the wider one with its top words zero).
+    bool one = multi;
+    int widest = 4;
+    for (const sigp_instance *inst : insts) {
+        one = one && inst->one_launch;
+        widest = std::max(widest, nw_bucket(inst->n));
+    }
+    auto class_of = [&](int k) { return one ? widest : nw_bucket(insts[k]->n); };
     std::vector<int32_t> *orders = pl.orders, *samples = pl.samples, *checks = pl.checks;
     // the autotuner times each mapping on about 2^19 simulations of a class,
     // and checks a close ranking on about 2^21
@@ -1661,7 +1672,7 @@ static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool mu
         LaunchUnit &u = bt->units[bt->n_units];
         std::vector<std::pair<float, int32_t>> items;
         for (int k = 0; k < K; ++k) {
-            if (nw_bucket(insts[k]->n) != classes[c]) continue;
+            if (class_of(k) != classes[c]) continue;
             u.max_n = std::max(u.max_n, insts[k]->n);
             const float nn = (float)insts[k]->n * (float)insts[k]->n;
             for (int p = 0; p < P; ++p) items.push_back({pr.cost[p] * nn, (int32_t)(k * P + p)});
@@ -1867,6 +1878,12 @@ static sigp_status check_config(const sigp_launch_config *cfg) {
 extern "C" sigp_status sigp_instance_skip_repeats(sigp_instance *inst, int32_t enable) {
     if (!inst) return fail(SIGP_INVALID_ARG, "instance is NULL");
     inst->skip_repeats = enable != 0;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_instance_one_launch(sigp_instance *inst, int32_t enable) {
+    if (!inst) return fail(SIGP_INVALID_ARG, "instance is NULL");
+    inst->one_launch = enable != 0;
     return SIGP_OK;
 }
 

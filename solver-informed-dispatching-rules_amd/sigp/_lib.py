@@ -22,7 +22,7 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_zstats", "sigp_evaluate", "sigp_evaluate_noalloc", "sigp_batch_create", "sigp_batch_run",
             "sigp_batch_destroy", "sigp_batch_shape", "sigp_device_math", "sigp_batch_configure",
             "sigp_batch_order_scenarios", "sigp_instance_skip_repeats",
-            "sigp_instance_protocol", "sigp_batch_protocol", "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
+            "sigp_instance_protocol", "sigp_batch_protocol", "sigp_instance_one_launch", "sigp_batch_autotune", "sigp_instance_configure", "sigp_batch_create_multi",
             "sigp_evaluate_multi", "sigp_phenotype", "sigp_mlp_create", "sigp_mlp_destroy",
             "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
             "sigp_mlp_predict", "sigp_gp_create", "sigp_gp_destroy", "sigp_gp_init", "sigp_gp_evaluate",
@@ -122,6 +122,7 @@ def lib() -> C.CDLL:
         "sigp_instance_skip_repeats": ([vp, C.c_int32], C.c_int),
         "sigp_instance_protocol": ([vp, C.c_int32], C.c_int),
         "sigp_batch_protocol": ([vp, C.c_int32], C.c_int),
+        "sigp_instance_one_launch": ([vp, C.c_int32], C.c_int),
         "sigp_batch_autotune": ([vp, vp, i64, vp, vp], C.c_int),
         "sigp_instance_configure": ([vp, vp], C.c_int),
         "sigp_batch_create_multi": ([vp, i32, vp, i32, vp, i32, i32, vp], C.c_int),

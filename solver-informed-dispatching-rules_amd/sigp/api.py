@@ -175,6 +175,12 @@ class FactoryContext:
         m <= RR_MAX_M)."""
         check(lib().sigp_instance_protocol(self._h, PROTOCOL_ROUND_ROBIN if round_robin else PROTOCOL_VEHICLES))
 
+    def one_launch(self, enable: bool) -> None:
+        """Multi-instance batches created later over instances that all have
+        this enabled run every size class in one launch
+        (sigp_instance_one_launch)."""
+        check(lib().sigp_instance_one_launch(self._h, 1 if enable else 0))
+
     def configure(self, waves_per_item: int, mapping: int = MAP_AUTO) -> None:
         """Default launch mapping of later batches and evaluate* calls."""
         cfg = LaunchConfig(waves_per_item, mapping)

@@ -76,3 +76,28 @@ def test_multi_instance_device_batch_autotuned_with_means(oracle):
     assert_bitwise(means.cpu().numpy(), np.array([sum(r.tolist()) / T for r in R]), "multi means")
     with pytest.raises(Exception):
         b.run(flat.data_ptr(), K * T - 1, out.data_ptr())   # too few rows: rejected on the host
+
+
+def test_multi_instance_one_launch(oracle):
+    """Every size class in one launch (sigp_instance_one_launch on every
+    instance): the same results as the per-class launches and the oracle,
+    under scenario, candidate and segmented lanes; with one instance not
+    asking, the batch keeps its per-class launches (same results again)."""
+    from sigp.api import FactoryContext, evaluate_multi
+    insts = _instances()
+    ctxs = [FactoryContext(i) for i in insts]
+    trees = random_rules(20, NS_TERMINALS + EXTRA, seed=37, min_depth=1, max_depth=5)
+    pk = pack_trees(trees)
+    mask = term_mask_from(NS_TERMINALS + EXTRA)
+    T = 70
+    blocks = [gen_durations(i.durations, T, 3000 + k, mode="gamma") for k, i in enumerate(insts)]
+    want = _expected(oracle, ctxs, insts, pk, blocks, T, mask)
+    for c in ctxs:
+        c.one_launch(True)
+    for m, w in [(0, 0), (1, 4), (2, 1), (3, 1)]:
+        print("one launch, mapping", m, "waves", w, flush=True)
+        ctxs[0].configure(w, m)
+        assert_bitwise(evaluate_multi(ctxs, pk, blocks, T, mask), want, "one launch mapping %d waves %d" % (m, w))
+    ctxs[0].configure(0, 0)
+    ctxs[2].one_launch(False)
+    assert_bitwise(evaluate_multi(ctxs, pk, blocks, T, mask), want, "per-class launches")
