@@ -281,7 +281,9 @@ def bench_timing(args):
         "metric": "GP training time, 30 seeds x 51 generations x 1,024 rules (timing.cu)",
         "value": round(train_s, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": round(train_s * 1000, 1), "higher_is_better": False, "scaling": "weak",
-        "vs_baseline": None if ref is None else round(ref / train_s, 2), "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None if ref is None else round(ref / train_s, 2), "dtype": "f64",
+        "data": "reference instance %s (the reference's own file), service times from the reference's "
+                "seeded streams" % name,
         "config": {"workload": "timing %s%s --seeds 30: the reference's training on the device (same "
                                "populations as its host loop)" % (name, (" --terms " + terms) if terms else ""),
                    "instance": name, "vocab": len(vocab), "parallelism": "1 GPU; 30 seeds side by side"},
