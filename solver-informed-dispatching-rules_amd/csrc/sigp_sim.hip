@@ -625,11 +625,9 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     int cur = 0, veh = 0;
     double ct = 0.0, total = 0.0;
     bool running = active && I.m > 0;
-    double *rr = nullptr;
     uint32_t open = 0;
     if (S.rr) {
-        rr = rr_rows(lds_rows, R, W, wv, S.rr, lane);
-        rr_clear(rr, I.m);
+        rr_clear(rr_rows(lds_rows, R, W, wv, S.rr, lane), I.m);
         open = (1u << I.m) - 1u;
     }
     // A lane that has closed vehicle `veh` holds until every running lane of
@@ -677,7 +675,7 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
                 total += I.score[best];
                 wset<NW>(vis, best);
             }
-            if (S.rr) running = rr_rotate(rr, open, veh, cur, ct);
+            if (S.rr) running = rr_rotate(rr_rows(lds_rows, R, W, wv, S.rr, lane), open, veh, cur, ct);
         }
         if (__builtin_amdgcn_ballot_w64(running && !held) == 0 && running) {
             ++veh;
@@ -726,8 +724,8 @@ __device__ __forceinline__ int block_argmax(double pri, int idx, int W, int wv, 
 // state is identical in every thread.  Suited to few scenarios per rule (the
 // reference's GP loop evaluates one per generation, engine/gp.hpp:32), where
 // scenario lanes would sit idle.  Dynamic LDS as in sim_kernel.
-template <int NW, bool RMOD, bool RARE>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 256)))
+template <int NW, bool RMOD, bool RARE, int MINW = 1>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(MINW)))
 void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     __shared__ double rpri[4];
     __shared__ int ridx[4];
@@ -763,11 +761,9 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
         if (threadIdx.x == 0) S.first[pair] = -1;
         return;
     }
-    double *rr = nullptr;
     uint32_t open = 0;
     if (S.rr) {   // route states of the round-robin protocol (same in every thread)
-        rr = rr_rows(lds_rows, R, W, wv, S.rr, lane);
-        rr_clear(rr, I.m);
+        rr_clear(rr_rows(lds_rows, R, W, wv, S.rr, lane), I.m);
         open = (1u << I.m) - 1u;
     }
     while (running) {
@@ -793,7 +789,7 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
                 total += I.score[best];
                 wset<NW>(vis, best);
             }
-            running = rr_rotate(rr, open, veh, cur, ct);
+            running = rr_rotate(rr_rows(lds_rows, R, W, wv, S.rr, lane), open, veh, cur, ct);
         } else if (best < 0) {
             ++veh;
             cur = 0;
@@ -899,11 +895,9 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
     int cur = 0, veh = 0;
     double ct = 0.0, total = 0.0;
     bool running = active && I.m > 0;
-    double *rr = nullptr;
     uint32_t open = 0;
     if (S.rr) {   // round robin, as in sim_kernel
-        rr = rr_rows(lds_rows, R, 1, 0, S.rr, lane);
-        rr_clear(rr, I.m);
+        rr_clear(rr_rows(lds_rows, R, 1, 0, S.rr, lane), I.m);
         open = (1u << I.m) - 1u;
     }
     // Vehicle boundaries are kept together as in sim_kernel: a simulation
@@ -934,7 +928,7 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
                 total += I.score[best];
                 wset<NW>(vis, best);
             }
-            if (S.rr) running = rr_rotate(rr, open, veh, cur, ct);
+            if (S.rr) running = rr_rotate(rr_rows(lds_rows, R, 1, 0, S.rr, lane), open, veh, cur, ct);
         }
         if (__builtin_amdgcn_ballot_w64(running && !held) == 0 && running) {
             ++veh;
@@ -1084,6 +1078,7 @@ struct sigp_batch {
     long long n_out = 0;            // results: instances x P x T
     bool has_rmod = false;          // some rule has an rmod node
     bool has_rare = false;          // some rule reads a slot outside the scan families
+    bool has_atrisk = false;        // some rule computes ATRISK (selects the candidate-lane variant)
     void *dbuf = nullptr;           // rules, instance slots and launch orders
     RuleSet rs{};                   // rule arrays (order: set per launch unit)
     cslot slots = nullptr;
@@ -1443,7 +1438,7 @@ struct PreparedRules {
     std::vector<float> cost;
     int max_depth = 1, max_k = 1;
     bool any_rmod = false;
-    uint64_t any_rd = 0;
+    uint64_t any_rd = 0, any_fl = 0;
 };
 
 static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, int32_t n_masks,
@@ -1515,6 +1510,7 @@ static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, 
         }
         pr.rd[p] = r;
         pr.any_rd |= r;
+        pr.any_fl |= f;
         pr.flags[p] = f;
         pr.cost[p] = rule_cost((int)L, f, c);
     }
@@ -1633,6 +1629,7 @@ static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool mu
     bt->n_out = (long long)K * P * T;
     bt->has_rmod = pr.any_rmod;
     bt->has_rare = (pr.any_rd & RARE_SLOTS) != 0;
+    bt->has_atrisk = (pr.any_fl & F_ATRISK) != 0;
     for (const sigp_instance *inst : insts)
         if (inst->protocol != insts[0]->protocol)
             return fail(SIGP_INVALID_ARG, "the instances of a batch must share one protocol (sigp_instance_protocol)");
@@ -1990,9 +1987,21 @@ static void launch_variant(const sigp_batch *bt, const LaunchUnit &u, int mappin
             return;
         }
     }
-    if (mapping != SIGP_MAP_SCENARIOS)
+    if (mapping != SIGP_MAP_SCENARIOS) {
+        // At n <= 128 the candidate-lane kernel takes 169 VGPRs, one above
+        // 3 wavefronts per SIMD.  Held to 3 (a 12-byte spill), it ran the
+        // reference's GP loops over rules without ATRISK 9-17 % faster (GA
+        // batch, timing tool with NSTW/MAXN/NFEAS/REACH5) and those reading
+        // ATRISK 4-5 % slower (random EXTRA-18 populations, the 18-terminal
+        // trace), so the batch's rules choose.
+        if constexpr (NW == 4) {
+            if (!bt->has_atrisk) {
+                sim_cand_kernel<NW, RMOD, RARE, 3><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
+                return;
+            }
+        }
         sim_cand_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
-    else
+    } else
         sim_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
 }
 
@@ -2552,6 +2561,7 @@ sigp_status sigp_internal_batch_device(sigp_instance *inst, const sigp_device_ru
     bt->n_out = (long long)r.P * T;
     bt->has_rmod = r.has_rmod;
     bt->has_rare = r.has_rare;
+    bt->has_atrisk = r.has_atrisk;
     InstSlot slot;
     slot.dev = inst->dev;
     slot.tabs = inst->tabs;
