@@ -433,6 +433,14 @@ sigp_status sigp_gp_step(sigp_gp *gp, const double *fitness);
  * then be NULL) detaches it, such a run takes plain steps: the next parents
  * are sub-step 0's offspring, the gp_step of the run's own stream. */
 sigp_status sigp_gp_set_surrogate(sigp_gp *gp, sigp_mlp *net, const sigp_situations *sit);
+/* Screened initial population (no counterpart in the reference, whose
+ * initial population is one ramped_half_and_half, engine/gp.hpp:67-77): for
+ * a run with offspring = k x population and a network attached, each of its
+ * k streams draws a ramped_half_and_half of population trees (stream 0's are
+ * what sigp_gp_init would give), and the screen of a step with no elites
+ * keeps population of the k x population: distinct phenotypes first, then
+ * higher score, then lower slot.  Replaces sigp_gp_init for such runs. */
+sigp_status sigp_gp_init_screened(sigp_gp *gp);
 /* Phenotypes of the current population (no counterpart in the reference):
  * ranks (n_runs*population x n_sit x n_cand int32, as sigp_phenotype) of
  * every tree on `sit`, computed from the device's own compiled trees.  With

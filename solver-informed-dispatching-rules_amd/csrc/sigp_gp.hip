@@ -161,22 +161,27 @@ __device__ __forceinline__ int generate(const GpParams &P, Rng &g, int depth, bo
 }
 
 // engine/gp.hpp:67-77: depths ramp min..max, alternating full/grow blocks.
-// One wavefront per run, on the run's first stream.
-__global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, int16_t *tok, int32_t *len) {
+// One wavefront per (run, stream): wavefront r * nsub + s draws pop trees
+// from run r's stream s into slots r * per_run + s * pop + i.  The
+// population: nsub = 1 (the run's first stream), per_run = pop.  The
+// screened initial pool: nsub = subs, per_run = n_off (sigp_gp_init_screened).
+__global__ __launch_bounds__(64) void init_kernel(GpParams P, uint32_t *state, int16_t *tok, int32_t *len,
+                                                  int nsub, int per_run) {
     __shared__ uint32_t mt[624];
-    const int r = (int)blockIdx.x;
+    const int r = (int)blockIdx.x / nsub, sub = (int)blockIdx.x - r * nsub;
+    uint32_t *st = state + ((size_t)r * P.subs + sub) * MT_WORDS;
     Rng g;
-    load_rng(g, mt, state + (size_t)r * P.subs * MT_WORDS);
+    load_rng(g, mt, st);
     const int nd = P.max_depth - P.min_depth + 1;
     for (int i = 0; i < P.pop; ++i) {
         const int d = P.min_depth + i % nd;
         const bool full = (i / nd) % 2 == 0;
         int h;
-        const size_t gi = (size_t)r * P.pop + i;
+        const size_t gi = (size_t)r * per_run + (size_t)sub * P.pop + i;
         const int n = generate(P, g, d, full, tok + gi * MAXL, h);
         if (threadIdx.x == 0) len[gi] = n;
     }
-    store_rng(g, state + (size_t)r * P.subs * MT_WORDS);
+    store_rng(g, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -791,6 +796,23 @@ static sigp_status launch_evaluation(sigp_gp *g) {
 // parents, then the new parents' analysis.  A loop created for screening
 // with no network attached takes sub-step 0's offspring, which is the plain
 // gp_step on the run's own stream.
+// The screen over the offspring slots: compilation, phenotypes, predicted
+// rewards, phenotypic duplicates and the choice of the next parents (the
+// first min(elitism, pop) slots of sub-step 0 kept as elites), into g->sel.
+static sigp_status launch_screen(sigp_gp *g, const GpParams &p) {
+    analyze_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(p, g->oset, g->mask);
+    sigp_status st = sigp_internal_phenotype_device(g->obt, g->sit, g->ranks, g->stream);
+    if (st != SIGP_OK) return st;
+    st = sigp_internal_mlp_predict_ranks(g->net, g->ranks, (int64_t)g->runs * p.n_off, g->sit.n_cand, g->pred,
+                                         g->stream);
+    if (st != SIGP_OK) return st;
+    phash_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(g->ranks, SIGP_MLP_IN, g->phash);
+    dup_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->phash, g->dup);
+    select_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->pred, g->dup, g->sel);
+    GP_TRY(hipGetLastError());
+    return SIGP_OK;
+}
+
 static sigp_status launch_step(sigp_gp *g) {
     const GpParams &p = g->prm;
     const size_t lds = (size_t)p.pop * (sizeof(double) + 4 + 1);
@@ -798,15 +820,8 @@ static sigp_status launch_step(sigp_gp *g) {
     assemble_kernel<<<dim3(p.n_off, g->runs), 64, 0, g->stream>>>(p, g->rec, g->ptok, g->grow, g->otok, g->olen);
     const int32_t *sel = nullptr;
     if (p.subs > 1 && g->net) {
-        analyze_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(p, g->oset, g->mask);
-        sigp_status st = sigp_internal_phenotype_device(g->obt, g->sit, g->ranks, g->stream);
+        const sigp_status st = launch_screen(g, p);
         if (st != SIGP_OK) return st;
-        st = sigp_internal_mlp_predict_ranks(g->net, g->ranks, (int64_t)g->runs * p.n_off, g->sit.n_cand, g->pred,
-                                             g->stream);
-        if (st != SIGP_OK) return st;
-        phash_kernel<<<g->runs * p.n_off, 64, 0, g->stream>>>(g->ranks, SIGP_MLP_IN, g->phash);
-        dup_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->phash, g->dup);
-        select_kernel<<<dim3((p.n_off + 255) / 256, g->runs), 256, 0, g->stream>>>(p, g->pred, g->dup, g->sel);
         sel = g->sel;
     }
     gather_kernel<<<dim3(p.pop, g->runs), 64, 0, g->stream>>>(p, sel, g->otok, g->olen, g->ptok, g->plen);
@@ -1120,7 +1135,27 @@ extern "C" sigp_status sigp_gp_destroy(sigp_gp *g) {
 extern "C" sigp_status sigp_gp_init(sigp_gp *g) {
     if (!g) return gp_fail("gp is NULL");
     // ramped half-and-half from each run's first stream
-    init_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->mt, g->ptok, g->plen);
+    init_kernel<<<g->runs, 64, 0, g->stream>>>(g->prm, g->mt, g->ptok, g->plen, 1, g->prm.pop);
+    launch_rebuild(g);
+    GP_TRY(hipGetLastError());
+    GP_TRY(hipStreamSynchronize(g->stream));
+    g->ready = true;
+    return SIGP_OK;
+}
+
+extern "C" sigp_status sigp_gp_init_screened(sigp_gp *g) {
+    if (!g) return gp_fail("gp is NULL");
+    if (g->prm.subs < 2 || !g->net) return gp_fail("a screened initial population needs a surrogate (sigp_gp_set_surrogate)");
+    // subs x pop ramped half-and-half trees per run, pop from each of its
+    // streams (stream 0's are the plain initial population), then the
+    // screen of a step with no elites
+    init_kernel<<<g->runs * g->prm.subs, 64, 0, g->stream>>>(g->prm, g->mt, g->otok, g->olen, g->prm.subs,
+                                                             g->prm.n_off);
+    GpParams p = g->prm;
+    p.elitism = 0;
+    sigp_status st = launch_screen(g, p);
+    if (st != SIGP_OK) return st;
+    gather_kernel<<<dim3(p.pop, g->runs), 64, 0, g->stream>>>(p, g->sel, g->otok, g->olen, g->ptok, g->plen);
     launch_rebuild(g);
     GP_TRY(hipGetLastError());
     GP_TRY(hipStreamSynchronize(g->stream));

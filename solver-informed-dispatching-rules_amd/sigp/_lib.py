@@ -27,7 +27,8 @@ EXPORTED = ["sigp_last_error", "sigp_device_info", "sigp_instance_create", "sigp
             "sigp_mlp_get_params", "sigp_mlp_set_params", "sigp_mlp_gradients", "sigp_mlp_train",
             "sigp_mlp_predict", "sigp_gp_create", "sigp_gp_destroy", "sigp_gp_init", "sigp_gp_evaluate",
             "sigp_gp_step", "sigp_gp_run", "sigp_gp_population", "sigp_gp_packed", "sigp_gp_rng_state",
-            "sigp_gp_set_surrogate", "sigp_gp_phenotype", "sigp_sample_durations", "sigp_sample_durations_async",
+            "sigp_gp_set_surrogate", "sigp_gp_init_screened", "sigp_gp_phenotype", "sigp_sample_durations",
+            "sigp_sample_durations_async",
             "sigp_gp_set_sampler"]
 
 _STATUS = {0: "SIGP_OK", 1: "SIGP_INVALID_ARG", 2: "SIGP_DEVICE_ERROR", 3: "SIGP_OOM",
@@ -142,6 +143,7 @@ def lib() -> C.CDLL:
         "sigp_gp_step": ([vp, vp], C.c_int),
         "sigp_gp_run": ([vp, vp, i64, i32, i32, vp, vp, vp, vp], C.c_int),
         "sigp_gp_set_surrogate": ([vp, vp, vp], C.c_int),
+        "sigp_gp_init_screened": ([vp], C.c_int),
         "sigp_gp_phenotype": ([vp, vp, vp], C.c_int),
         "sigp_gp_set_sampler": ([vp, vp, vp, i64, i64], C.c_int),
         "sigp_sample_durations": ([vp, vp, i64, i64, vp], C.c_int),

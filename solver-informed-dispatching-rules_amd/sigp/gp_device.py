@@ -178,9 +178,12 @@ class DeviceGP:
             raise ValueError("need one seed per run")
         check(lib().sigp_gp_set_sampler(self._h, C.byref(smp), _ptr(seeds), int(gen_stride), int(gen_offset)))
 
-    def init(self) -> None:
-        """ramped_half_and_half of every run (engine/gp.hpp:67-77)."""
-        check(lib().sigp_gp_init(self._h))
+    def init(self, screened: bool = False) -> None:
+        """ramped_half_and_half of every run (engine/gp.hpp:67-77); with
+        screened=True (a screen attached), k such populations per run, one
+        per stream, of which the screen keeps the population
+        (sigp_gp_init_screened)."""
+        check(lib().sigp_gp_init_screened(self._h) if screened else lib().sigp_gp_init(self._h))
 
     def _rows(self, sampled, rows: int) -> np.ndarray:
         a = np.ascontiguousarray(sampled, dtype=np.float64).reshape(-1)

@@ -168,7 +168,7 @@ def screening_gain(sc: Scorer, net: Mlp, norm, pool: int, seed: int):
 
 
 def run_device_gp(sc: Scorer, net, generations: int, pop: int, breed: int, seed: int, label: str,
-                  chunk: int = 5, launch=None):
+                  chunk: int = 5, launch=None, screened_init: bool = False):
     """One GP loop on the device (one HIP graph per generation): plain, or
     screened by `net` when given (breed = k x pop offspring per generation),
     run in calls of `chunk` generations.  Returns the held-out reward of each
@@ -182,8 +182,10 @@ def run_device_gp(sc: Scorer, net, generations: int, pop: int, breed: int, seed:
                   offspring=breed if net is not None else 0, **(launch or {}))
     if net is not None:
         gp.set_surrogate(net, sc.sit)
-    gp.init()
-    best, gms, t_dev = [], [], 0.0
+    t0 = time.perf_counter()
+    gp.init(screened=screened_init and net is not None)
+    init_ms = 1000.0 * (time.perf_counter() - t0) if screened_init and net is not None else 0.0
+    best, gms, t_dev = [], [], init_ms
     for g0 in range(0, generations, chunk):
         k = min(chunk, generations - g0)
         rows = np.concatenate([sc.train_scenarios(g) for g in range(g0, g0 + k)])
@@ -309,6 +311,9 @@ def main(argv=None):
     ap.add_argument("--breed", type=int, default=0, help="offspring bred per generation (0: 8 x population)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--gp-runs", type=int, default=5, help="GP seed pairs (plain, surrogate) with one network")
+    ap.add_argument("--screened-init", action="store_true",
+                    help="surrogate runs start from a screened initial population (k ramped half-and-half "
+                         "populations, the network keeps one; its time counts in the run)")
     ap.add_argument("--online", action="store_true",
                     help="train each surrogate run's network inside the run from its own exact evaluations "
                          "(no labelling setup) instead of one network trained beforehand")
@@ -359,7 +364,7 @@ def main(argv=None):
                                             launch=launch)
             else:
                 surr = run_device_gp(sc, net, a.generations, a.pop, a.breed, gseed, "surrogate seed %d" % gseed,
-                                     launch=launch)
+                                     launch=launch, screened_init=a.screened_init)
             target = plain["held_out_best_per_generation"][-1]
             pg, pms = reach(plain["held_out_best_per_generation"], plain["generation_ms"], target)
             sgen, sms = reach(surr["held_out_best_per_generation"], surr["generation_ms"], target)
@@ -373,6 +378,7 @@ def main(argv=None):
                  "plain run %.0f ms, setup share %.0f ms"
                  % (gseed, target, pg, pms, sgen, sms if sms is not None else float("nan"), total, share))
         out["gp"] = {"generations": a.generations, "pop": a.pop, "breed": a.breed, "runs": runs,
+                     "screened_init": bool(a.screened_init),
                      "network_setup_s": round(setup_s, 3),
                      "runs_sharing_the_network": 1 if a.online else a.gp_runs}
         for key in ("time_ratio", "time_ratio_with_setup", "time_ratio_to_plain_reach",

@@ -211,6 +211,56 @@ def test_device_screened_steps_match_host_restatement():
         print("  distinct phenotypes among %d offspring: %d" % (k * pop, k * pop - sum(dup)), flush=True)
 
 
+def test_device_screened_init_matches_host_restatement():
+    """Screened initial population (no reference counterpart): k ramped
+    half-and-half populations, one per stream Mt19937(seed + s * 2^32), the
+    screen keeping pop of the k * pop trees with no elites -- against the same
+    restated on the host; then one screened step from it."""
+    from sigp.api import FactoryContext
+    from sigp.gp_device import SUB_STREAM, DeviceGP
+    from sigp.surrogate import Mlp, features, make_situations, phenotype
+    inst = load_instance("r101")
+    ctx = FactoryContext(inst)
+    terms = list(NS_TERMINALS) + list(EXTRA)
+    mask = term_mask_from(terms)
+    pop, k, seed = 48, 4, 19
+    tc = G.TreeConfig(terms, list(FUNCTIONS), max_depth=6)
+    ec = G.EvolutionConfig(population_size=pop, elitism=2, max_depth=7)
+    gp = DeviceGP(ctx, tc, ec, seeds=[seed], T=2, term_mask=mask, offspring=k * pop)
+    net = Mlp(seed=4)
+    sit = make_situations(inst, seed=6)
+    gp.set_surrogate(net, sit)
+    gp.init(screened=True)
+    rngs = [random.Random(seed + s * SUB_STREAM) for s in range(k)]
+
+    def screen(trees, E):
+        ranks = phenotype(ctx, pack_trees(trees), mask, sit)
+        pred = net.predict(features(ranks))
+        seen, dup = set(), []
+        for j in range(len(trees)):
+            h = _fnv1a(ranks[j])
+            dup.append(h in seen)
+            seen.add(h)
+        cand = [j for j in range(len(trees)) if j % pop >= E]
+        key = lambda j: (dup[j], -float(pred[j]) if pred[j] == pred[j] else float("inf"), j)
+        return trees[:E] + [trees[j] for j in sorted(cand, key=key)[:pop - E]]
+
+    pool = []
+    for s in range(k):
+        pool += G.ramped_half_and_half(tc, rngs[s], pop)
+    host = screen(pool, 0)
+    assert gp.prefixes(0) == [to_prefix(t) for t in host]
+    for s in range(k):
+        assert gp.rng_state(0, s) == _state(rngs[s]), s
+    fit = np.random.default_rng(2).integers(0, 9, size=pop).astype(np.float64)
+    gp.step(fit[None, :])
+    off = []
+    for s in range(k):
+        off += G.gp_step(host, list(fit), rngs[s], tc, ec)
+    host = screen(off, min(ec.elitism, pop))
+    assert gp.prefixes(0) == [to_prefix(t) for t in host]
+
+
 def _fnv1a(ranks):
     h = 0xCBF29CE484222325
     for v in np.asarray(ranks).ravel():
