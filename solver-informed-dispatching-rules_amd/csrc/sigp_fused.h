@@ -32,6 +32,30 @@ struct FusedTabs {
     cdp open, close, dur, d2d, score, durrob, sdv;   // customer arrays, padded to ldp
 };
 
+// A kernel's copy of an instance's tables.  The customer arrays are the
+// DevInst's own (the same allocation).  SHARED takes those pointers from
+// `dev` rather than loading the copies in `tabs`, which leaves one scalar
+// register pair per array instead of two.  The kernels sit at the 102-SGPR
+// limit, and this moves their spills: measured on MI355X, the scenario-lane
+// kernel at n <= 128 ran 5 % faster with it (plain workload 67.7 -> 64.2 ms),
+// the one at n <= 288 3 % slower (1,846 -> 1,888 ms on 512 rules x 512
+// scenarios of the large workload) and the candidate-lane kernels no faster,
+// so only the n <= 128 scenario-lane kernel takes it.
+template <bool SHARED>
+__device__ __forceinline__ FusedTabs tabs_of(const FusedTabs &tabs, const DevInst &dev) {
+    FusedTabs F = tabs;
+    if constexpr (SHARED) {
+        F.open = dev.open;
+        F.close = dev.close;
+        F.dur = dev.dur;
+        F.d2d = dev.d2d;
+        F.score = dev.score;
+        F.durrob = dev.durrob;
+        F.sdv = dev.sdv;
+    }
+    return F;
+}
+
 __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
 
 struct D8 { double v[8]; };

@@ -595,7 +595,7 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     const int p = pair - k * R.P;
     if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
     const DevInst I = slots[k].dev;
-    const FusedTabs F = slots[k].tabs;
+    const FusedTabs F = tabs_of<NW == 4>(slots[k].tabs, I);
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -737,7 +737,7 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
     const int p = pair - k * R.P;
     if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
     const DevInst I = slots[k].dev;
-    const FusedTabs F = slots[k].tabs;
+    const FusedTabs F = tabs_of<false>(slots[k].tabs, I);
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
@@ -869,7 +869,7 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
     const int p = pair - k * R.P;
     if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
     const DevInst I = slots[k].dev;
-    const FusedTabs F = slots[k].tabs;
+    const FusedTabs F = tabs_of<false>(slots[k].tabs, I);
     const int lane = threadIdx.x & 63;
     const int sl = lane % SEGW;
     int s = blk * SEGS + lane / SEGW;
@@ -966,7 +966,7 @@ void pheno_kernel(cslot slots, RuleSet R, PhenoSpec Ph, int32_t *__restrict__ ou
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
     const DevInst I = slots[0].dev;
-    const FusedTabs F = slots[0].tabs;
+    const FusedTabs F = tabs_of<false>(slots[0].tabs, I);
     const int off = R.off[p], L = R.len[p];
     const cip ops = R.ops + off;
     const cip args = R.args + 2 * (size_t)off;
