@@ -261,7 +261,9 @@ sigp_status sigp_instance_one_launch(sigp_instance *inst, int32_t enable);
  * sigp_instance_protocol sets the protocol of batches created later on the
  * instance, including the ones sigp_evaluate and sigp_evaluate_noalloc
  * create; a multi-instance batch needs its instances to agree.
- * sigp_batch_protocol switches an existing batch. */
+ * sigp_batch_protocol switches an existing batch.  The device GP loop
+ * (sigp_gp_*) evaluates with SIGP_PROTOCOL_VEHICLES, as the reference's GP
+ * engine does. */
 #define SIGP_PROTOCOL_VEHICLES 0
 #define SIGP_PROTOCOL_ROUND_ROBIN 1
 #define SIGP_RR_MAX_M 8
