@@ -44,6 +44,9 @@ struct FusedTabs {
 template <bool SHARED>
 __device__ __forceinline__ FusedTabs tabs_of(const FusedTabs &tabs, const DevInst &dev) {
     FusedTabs F = tabs;
+    const cbase b = tabs.t0.base;   // the threshold tables' allocation
+    F.t0.base = b; F.t0n.base = b; F.t2.base = b; F.t2n.base = b; F.r.base = b; F.d.base = b;
+    F.t0c.base = b; F.t0nc.base = b; F.t1c.base = b; F.t2c.base = b; F.t2nc.base = b; F.rc.base = b;
     if constexpr (SHARED) {
         F.open = dev.open;
         F.close = dev.close;
@@ -52,6 +55,10 @@ __device__ __forceinline__ FusedTabs tabs_of(const FusedTabs &tabs, const DevIns
         F.score = dev.score;
         F.durrob = dev.durrob;
         F.sdv = dev.sdv;
+    } else {
+        const cbase c = dev.dist.base;   // the customer arrays live in the instance's allocation
+        F.open.base = c; F.close.base = c; F.dur.base = c; F.d2d.base = c; F.score.base = c;
+        F.durrob.base = c; F.sdv.base = c;
     }
     return F;
 }
@@ -60,9 +67,13 @@ __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a
 
 struct D8 { double v[8]; };
 __device__ __forceinline__ D8 ld8(cdp row, int j0) {
+    // one address for the chunk, so that the eight reads merge into one
+    // 64-byte load (offsets in 32 bits cannot be shown not to wrap)
+    const __attribute__((address_space(4))) double *p =
+        (const __attribute__((address_space(4))) double *)(row.base + (row.off + 8u * (uint32_t)j0));
     D8 r;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) r.v[k] = row[j0 + k];
+    for (int k = 0; k < 8; ++k) r.v[k] = p[k];
     return r;
 }
 

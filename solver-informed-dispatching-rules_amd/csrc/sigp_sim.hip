@@ -594,7 +594,7 @@ void sim_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     const int k = pair / R.P;
     const int p = pair - k * R.P;
     if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
-    const DevInst I = slots[k].dev;
+    const DevInst I = dev_of(slots[k].dev);
     const FusedTabs F = tabs_of<NW == 4>(slots[k].tabs, I);
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -736,7 +736,7 @@ void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ ou
     const int k = pair / R.P;
     const int p = pair - k * R.P;
     if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
-    const DevInst I = slots[k].dev;
+    const DevInst I = dev_of(slots[k].dev);
     const FusedTabs F = tabs_of<false>(slots[k].tabs, I);
     const int W = (int)(blockDim.x >> 6);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -868,7 +868,7 @@ void sim_seg_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out
     const int k = pair / R.P;
     const int p = pair - k * R.P;
     if (S.rep && S.rep[p] != p) return;   // a repeated rule (whole workgroup)
-    const DevInst I = slots[k].dev;
+    const DevInst I = dev_of(slots[k].dev);
     const FusedTabs F = tabs_of<false>(slots[k].tabs, I);
     const int lane = threadIdx.x & 63;
     const int sl = lane % SEGW;
@@ -965,7 +965,7 @@ void pheno_kernel(cslot slots, RuleSet R, PhenoSpec Ph, int32_t *__restrict__ ou
     const int p = (int)blockIdx.x;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = threadIdx.x & 63;
-    const DevInst I = slots[0].dev;
+    const DevInst I = dev_of(slots[0].dev);
     const FusedTabs F = tabs_of<false>(slots[0].tabs, I);
     const int off = R.off[p], L = R.len[p];
     const cip ops = R.ops + off;
@@ -1307,7 +1307,7 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     D.ld = ld;
     D.K = K;
     D.tmax = d->tmax;
-    auto cd = [&](size_t o) { return (cdp)(base + o); };
+    auto cd = [&](size_t o) { return cdp{(cbase)base, (uint32_t)o}; };
     D.dist = cd(o_dist);
     D.distT = cd(o_distT);
     D.rk = cd(o_rk);
@@ -1319,7 +1319,7 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
     D.score = cd(o_score);
     D.durrob = cd(o_durrob);
     D.sdv = cd(o_sdv);
-    D.dcls = (cip)(base + o_cls);
+    D.dcls = cid{(cbase)base, (uint32_t)o_cls};
     D.pw2 = cd(o_pw2);
     D.pw1 = cd(o_pw1);
     D.atp1 = cd(o_atp1);
@@ -1360,10 +1360,11 @@ static sigp_status build_instance(const sigp_instance_desc *d, sigp_instance *I)
             (const double *)(base + o_score), d->tmax, t[0], t[1], t[2], t[3], t[4], t[5], t[6],
             t[7], t[8], t[9], t[10], t[11]);
         HIP_TRY(hipGetLastError());
-        F.t0 = (cdp)t[0]; F.t0n = (cdp)t[1]; F.t2 = (cdp)t[2]; F.t2n = (cdp)t[3];
-        F.r = (cdp)t[4]; F.d = (cdp)t[5];
-        F.t0c = (cdp)t[6]; F.t0nc = (cdp)t[7]; F.t1c = (cdp)t[8]; F.t2c = (cdp)t[9];
-        F.t2nc = (cdp)t[10]; F.rc = (cdp)t[11];
+        auto ct = [&](int k) { return cdp{(cbase)tb, (uint32_t)((char *)t[k] - tb)}; };
+        F.t0 = ct(0); F.t0n = ct(1); F.t2 = ct(2); F.t2n = ct(3);
+        F.r = ct(4); F.d = ct(5);
+        F.t0c = ct(6); F.t0nc = ct(7); F.t1c = ct(8); F.t2c = ct(9);
+        F.t2nc = ct(10); F.rc = ct(11);
         F.open = D.open; F.close = D.close; F.dur = D.dur; F.d2d = D.d2d; F.score = D.score;
         F.durrob = D.durrob; F.sdv = D.sdv;
     }
@@ -2539,7 +2540,9 @@ extern "C" sigp_status sigp_device_math(int32_t which, const double *x, const do
 // interfaces for the library's other translation units
 
 int sigp_internal_instance_n(const sigp_instance *inst) { return inst->n; }
-const double *sigp_internal_instance_dur(const sigp_instance *inst) { return (const double *)(uintptr_t)inst->dev.dur; }
+const double *sigp_internal_instance_dur(const sigp_instance *inst) {
+    return (const double *)((uintptr_t)inst->dev.dur.base + inst->dev.dur.off);
+}
 
 sigp_status sigp_internal_batch_device(sigp_instance *inst, const sigp_device_rules &r, int T, int group_size,
                                        const sigp_launch_config &cfg, sigp_batch **out) {

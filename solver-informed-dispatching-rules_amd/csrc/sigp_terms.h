@@ -23,8 +23,30 @@ namespace sigp {
 
 // Read-only instance memory in the constant address space: uniform loads
 // become scalar (SMEM) loads.
-typedef const __attribute__((address_space(4))) double *cdp;
 typedef const __attribute__((address_space(4))) int32_t *cip;
+typedef const __attribute__((address_space(4))) char *cbase;
+
+// An instance table: a 32-bit byte offset from the base address of the
+// allocation that holds it.  Kernels give every table of one allocation the
+// same base value (dev_of, tabs_of), so the tables cost one scalar register
+// each plus one pair per allocation, instead of a pair each, and scalar
+// loads take the SGPR-offset form.  An instance's two allocations stay far
+// below 4 GiB (n <= 512: about 8.5 MB and 25 MB).
+struct cdp {
+    cbase base;
+    uint32_t off;
+    __host__ __device__ __forceinline__ cdp operator+(uint32_t k) const { return cdp{base, off + 8u * k}; }
+    __device__ __forceinline__ double operator[](uint32_t i) const {
+        return *(const __attribute__((address_space(4))) double *)(base + (off + 8u * i));
+    }
+};
+struct cid {
+    cbase base;
+    uint32_t off;
+    __device__ __forceinline__ int32_t operator[](uint32_t i) const {
+        return *(const __attribute__((address_space(4))) int32_t *)(base + (off + 4u * i));
+    }
+};
 typedef const __attribute__((address_space(4))) uint64_t *cup;
 
 constexpr double kSigma = 0.2;                  // engine/sim_terms.hpp:15
@@ -40,7 +62,7 @@ struct DevInst {
     cdp open, close, dur, d2d, score;
     cdp durrob;  // dur * (1 + sigma)
     cdp sdv;     // 0.2 * dur
-    cip dcls;    // duration class of each customer
+    cid dcls;    // duration class of each customer
     cdp pw2;     // K x K: pow(sd_c*sd_c + sd_j*sd_j, 0.5)
     cdp pw1;     // K: pow(0.0 + sd_j*sd_j, 0.5)
     // ATRISK saturation thresholds (engine/sim_terms.hpp:221-252): for a
@@ -58,6 +80,18 @@ struct DevInst {
     cdp atpb0;
     cdp zp;      // 10 z-statistics
 };
+
+// A kernel's view of an instance record: every table on the base of the
+// first (they share one allocation).
+__device__ __forceinline__ DevInst dev_of(const DevInst &src) {
+    DevInst I = src;
+    const cbase b = src.dist.base;
+    I.dist.base = b; I.distT.base = b; I.rk.base = b; I.rkT.base = b;
+    I.open.base = b; I.close.base = b; I.dur.base = b; I.d2d.base = b; I.score.base = b;
+    I.durrob.base = b; I.sdv.base = b; I.dcls.base = b; I.pw2.base = b; I.pw1.base = b;
+    I.atp1.base = b; I.atp0.base = b; I.atq2.base = b; I.atq1.base = b; I.atpb0.base = b; I.zp.base = b;
+    return I;
+}
 
 template <int NW>
 __device__ __forceinline__ uint32_t wsel(const uint32_t (&a)[NW], int w) {
