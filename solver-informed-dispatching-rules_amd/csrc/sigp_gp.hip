@@ -750,7 +750,6 @@ struct sigp_gp {
     sigp_batch *pbt = nullptr;        // the parents' phenotype (sigp_gp_phenotype; made on first use)
     int KL = 1, SL = 1;               // the batches' value-store and stack rows
     bool rmod = false, rare = false;  // and their kernel variants
-    bool atrisk = false;              // the vocabulary computes ATRISK (candidate-lane variant)
     sigp_mlp *net = nullptr;          // the screen (not owned)
     char *sbuf = nullptr;             // the decision situations
     sigp_device_situations sit{};
@@ -847,7 +846,6 @@ static sigp_status make_rules_batch(sigp_gp *g, const TreeSet &S, int P, const i
     R.SL = SL;
     R.has_rmod = rmod;
     R.has_rare = rare;
-    R.has_atrisk = g->atrisk;
     return sigp_internal_batch_device(g->inst, R, T, group_size, cfg, out);
 }
 
@@ -1019,7 +1017,6 @@ extern "C" sigp_status sigp_gp_create(sigp_instance *inst, const sigp_gp_config 
     g->SL = SL;
     g->rmod = rmod;
     g->rare = rare;
-    g->atrisk = (fl_all & F_ATRISK) != 0;
     if (status == SIGP_OK)
         status = make_rules_batch(g, S, g->P, g->order, KL, SL, rmod, rare, g->T, c->population, lc, &g->bt, g->rep);
     if (status == SIGP_OK && screened)

@@ -22,7 +22,7 @@ struct sigp_device_rules {
     // every rule is simulated.
     const int32_t *rep = nullptr;
     int P = 0, KL = 1, SL = 1;
-    bool has_rmod = false, has_rare = false, has_atrisk = false;
+    bool has_rmod = false, has_rare = false;
 };
 
 // Decision situations of the phenotype, resident on the device.

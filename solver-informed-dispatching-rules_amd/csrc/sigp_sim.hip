@@ -724,8 +724,8 @@ __device__ __forceinline__ int block_argmax(double pri, int idx, int W, int wv, 
 // state is identical in every thread.  Suited to few scenarios per rule (the
 // reference's GP loop evaluates one per generation, engine/gp.hpp:32), where
 // scenario lanes would sit idle.  Dynamic LDS as in sim_kernel.
-template <int NW, bool RMOD, bool RARE, int MINW = 1>
-__global__ __attribute__((amdgpu_flat_work_group_size(64, 256), amdgpu_waves_per_eu(MINW)))
+template <int NW, bool RMOD, bool RARE>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 256)))
 void sim_cand_kernel(cslot slots, RuleSet R, ScenSpec S, double *__restrict__ out) {
     __shared__ double rpri[4];
     __shared__ int ridx[4];
@@ -1078,7 +1078,6 @@ struct sigp_batch {
     long long n_out = 0;            // results: instances x P x T
     bool has_rmod = false;          // some rule has an rmod node
     bool has_rare = false;          // some rule reads a slot outside the scan families
-    bool has_atrisk = false;        // some rule computes ATRISK (selects the candidate-lane variant)
     void *dbuf = nullptr;           // rules, instance slots and launch orders
     RuleSet rs{};                   // rule arrays (order: set per launch unit)
     cslot slots = nullptr;
@@ -1439,7 +1438,7 @@ struct PreparedRules {
     std::vector<float> cost;
     int max_depth = 1, max_k = 1;
     bool any_rmod = false;
-    uint64_t any_rd = 0, any_fl = 0;
+    uint64_t any_rd = 0;
 };
 
 static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, int32_t n_masks,
@@ -1511,7 +1510,6 @@ static sigp_status prepare_rules(const sigp_rules *rules, const int64_t *masks, 
         }
         pr.rd[p] = r;
         pr.any_rd |= r;
-        pr.any_fl |= f;
         pr.flags[p] = f;
         pr.cost[p] = rule_cost((int)L, f, c);
     }
@@ -1630,7 +1628,6 @@ static sigp_status plan_batch(const std::vector<sigp_instance *> &insts, bool mu
     bt->n_out = (long long)K * P * T;
     bt->has_rmod = pr.any_rmod;
     bt->has_rare = (pr.any_rd & RARE_SLOTS) != 0;
-    bt->has_atrisk = (pr.any_fl & F_ATRISK) != 0;
     for (const sigp_instance *inst : insts)
         if (inst->protocol != insts[0]->protocol)
             return fail(SIGP_INVALID_ARG, "the instances of a batch must share one protocol (sigp_instance_protocol)");
@@ -1988,21 +1985,9 @@ static void launch_variant(const sigp_batch *bt, const LaunchUnit &u, int mappin
             return;
         }
     }
-    if (mapping != SIGP_MAP_SCENARIOS) {
-        // At n <= 128 the candidate-lane kernel takes 169 VGPRs, one above
-        // 3 wavefronts per SIMD.  Held to 3 (a 12-byte spill), it ran the
-        // reference's GP loops over rules without ATRISK 9-17 % faster (GA
-        // batch, timing tool with NSTW/MAXN/NFEAS/REACH5) and those reading
-        // ATRISK 4-5 % slower (random EXTRA-18 populations, the 18-terminal
-        // trace), so the batch's rules choose.
-        if constexpr (NW == 4) {
-            if (!bt->has_atrisk) {
-                sim_cand_kernel<NW, RMOD, RARE, 3><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
-                return;
-            }
-        }
+    if (mapping != SIGP_MAP_SCENARIOS)
         sim_cand_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
-    } else
+    else
         sim_kernel<NW, RMOD, RARE><<<items, 64 * w, lds, stream>>>(bt->slots, R, S, out);
 }
 
@@ -2564,7 +2549,6 @@ sigp_status sigp_internal_batch_device(sigp_instance *inst, const sigp_device_ru
     bt->n_out = (long long)r.P * T;
     bt->has_rmod = r.has_rmod;
     bt->has_rare = r.has_rare;
-    bt->has_atrisk = r.has_atrisk;
     InstSlot slot;
     slot.dev = inst->dev;
     slot.tabs = inst->tabs;
