@@ -89,6 +89,14 @@ __device__ __forceinline__ uint32_t bits8(const uint32_t (&a)[NW], int j0) {
     return (wsel<NW>(a, j0 >> 5) >> (j0 & 31)) & 0xffu;
 }
 
+// The same bits with candidate c added to the set: the scans skip c itself,
+// and one OR per chunk takes the place of a compare per neighbour.
+template <int NW>
+__device__ __forceinline__ uint32_t bits8c(const uint32_t (&a)[NW], int j0, int c) {
+    const uint32_t d = (uint32_t)(c - j0);
+    return bits8<NW>(a, j0) | (d < 8u ? 1u << d : 0u);
+}
+
 // NS (engine/sim_terms.hpp:18-34), MAXN (:76-93) and the first leg of
 // REACH2 (:101-111) from candidate c at time fin, one pass.
 template <int NW, bool NS, bool MX, bool R2>
@@ -98,12 +106,12 @@ __device__ __forceinline__ void scan_nsfam(const FusedTabs &F, int n, int c, dou
     double s = 0.0, m = 0.0, b = 0.0;
     int bi = -1;
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8<NW>(vis, j0);
+        const uint32_t vb = bits8c<NW>(vis, j0, c);
         const D8 T = ld8(tr, j0), R = ld8(rr, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int j = j0 + k;
-            const bool ok = !((vb >> k) & 1u) && j != c && fin <= T.v[k];
+            const bool ok = !((vb >> k) & 1u) && fin <= T.v[k];
             const double r = R.v[k];
             if (NS) s = sel(ok, s + r, s);
             if (MX) m = sel(ok && r > m, r, m);
@@ -127,14 +135,14 @@ __device__ __forceinline__ void scan_robfam(const FusedTabs &F, int n, int c, do
     double s = 0.0, b = 0.0, bk = -1.0;
     int bi = -1, ki = -1;
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8<NW>(vis, j0);
+        const uint32_t vb = bits8c<NW>(vis, j0, c);
         const D8 T = ld8(tn, j0), R = ld8(rr, j0);
         D8 TP;
         if (RKR) TP = ld8(tp, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int j = j0 + k;
-            const bool open = !((vb >> k) & 1u) && j != c;
+            const bool open = !((vb >> k) & 1u);
             const bool ok = open && frob <= T.v[k];
             const double r = R.v[k];
             if (NSR) s = sel(ok, s + r, s);
@@ -161,12 +169,11 @@ __device__ __forceinline__ double scan_regret(const FusedTabs &F, int n, int c, 
     const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp;
     double s = 0.0;
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t mb = bits8<NW>(fnow, j0) & ~bits8<NW>(vis, j0);
+        const uint32_t mb = bits8<NW>(fnow, j0) & ~bits8c<NW>(vis, j0, c);
         const D8 T = ld8(tr, j0), S = ld8(F.score, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int j = j0 + k;
-            const bool add = ((mb >> k) & 1u) && j != c && !(fin <= T.v[k]);
+            const bool add = ((mb >> k) & 1u) && !(fin <= T.v[k]);
             s = sel(add, s + S.v[k], s);
         }
     }
@@ -184,13 +191,13 @@ __device__ __forceinline__ void scan_struct(const FusedTabs &F, int n, int c, do
     double cnt = 0.0, sc = 0.0, ds = 1e18, bv = -1.0;
     int bi = -1;
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8<NW>(vis, j0), fb = bits8<NW>(fnow, j0);
+        const uint32_t vb = bits8c<NW>(vis, j0, c), fb = bits8<NW>(fnow, j0);
         const D8 T = ld8(tr, j0), R = ld8(rr, j0), D = ld8(dd, j0), S = ld8(F.score, j0),
                  C = ld8(F.close, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int q = j0 + k;
-            const bool open = !((vb >> k) & 1u) && q != c;
+            const bool open = !((vb >> k) & 1u);
             const bool after = fin_c <= T.v[k];
             const bool frag = open && ((fb >> k) & 1u) && !after;
             cnt = sel(frag, cnt + 1.0, cnt);
@@ -213,12 +220,12 @@ __device__ __forceinline__ int scan_reach1(const FusedTabs &F, int n, int c, dou
     double bv = -1.0;
     int bi = -1;
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8<NW>(vis, j0);
+        const uint32_t vb = bits8c<NW>(vis, j0, c);
         const D8 T = ld8(tr, j0), R = ld8(rr, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int q = j0 + k;
-            const bool take = !((vb >> k) & 1u) && q != c && t <= T.v[k] && R.v[k] > bv;
+            const bool take = !((vb >> k) & 1u) && t <= T.v[k] && R.v[k] > bv;
             bv = sel(take, R.v[k], bv);
             bi = take ? q : bi;
         }
@@ -235,12 +242,12 @@ __device__ __forceinline__ double scan_nstw(const FusedTabs &F, int n, int c, do
     const cdp tr = F.t0n + row, rr = F.r + row, dd = F.d + row;
     double s = 0.0;
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8<NW>(vis, j0);
+        const uint32_t vb = bits8c<NW>(vis, j0, c);
         const D8 T = ld8(tr, j0);
         uint32_t okm = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const bool ok = !((vb >> k) & 1u) && j0 + k != c && fin <= T.v[k];
+            const bool ok = !((vb >> k) & 1u) && fin <= T.v[k];
             okm |= (ok ? 1u : 0u) << k;
         }
         if (__builtin_amdgcn_ballot_w64(okm != 0) == 0) continue;
@@ -282,8 +289,11 @@ __device__ __forceinline__ double scan_eloss(const DevInst &I, const FusedTabs &
     uint32_t mb = 0;
 #pragma unroll 1
     for (int j = 0; j < I.n; ++j) {
-        if ((j & 31) == 0) mb = wsel<NW>(cnow, j >> 5) & ~wsel<NW>(vis, j >> 5);
-        const bool act = j != c && ((mb >> (j & 31)) & 1u);
+        if ((j & 31) == 0) {
+            const uint32_t d = (uint32_t)(c - j);   // c itself is skipped
+            mb = wsel<NW>(cnow, j >> 5) & ~(wsel<NW>(vis, j >> 5) | (d < 32u ? 1u << d : 0u));
+        }
+        const bool act = (mb >> (j & 31)) & 1u;
         if (__builtin_amdgcn_ballot_w64(act) == 0) continue;
         if (!act) continue;
         const double ma = fin_c + dd[j];
