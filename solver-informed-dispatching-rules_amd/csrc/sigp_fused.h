@@ -77,6 +77,16 @@ __device__ __forceinline__ D8 ld8(cdp row, int j0) {
     return r;
 }
 
+struct D4 { double v[4]; };
+__device__ __forceinline__ D4 ld4(cdp row, int j0) {
+    const __attribute__((address_space(4))) double *p =
+        (const __attribute__((address_space(4))) double *)(row.base + (row.off + 8u * (uint32_t)j0));
+    D4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.v[k] = p[k];
+    return r;
+}
+
 // True when every active lane holds the same value of v.
 __device__ __forceinline__ bool wave_uniform(int v) {
     const int v0 = __builtin_amdgcn_readfirstlane(v);
@@ -190,12 +200,16 @@ __device__ __forceinline__ void scan_struct(const FusedTabs &F, int n, int c, do
     const cdp tr = F.t0 + row, rr = F.r + row, dd = F.d + row;
     double cnt = 0.0, sc = 0.0, ds = 1e18, bv = -1.0;
     int bi = -1;
-    for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8c<NW>(vis, j0, c), fb = bits8<NW>(fnow, j0);
-        const D8 T = ld8(tr, j0), R = ld8(rr, j0), D = ld8(dd, j0), S = ld8(F.score, j0),
-                 C = ld8(F.close, j0);
+    // Four neighbours per chunk: five streams of eight would hold 80 scalar
+    // registers and spill inside the loop.
+    for (int j0 = 0; j0 < n; j0 += 4) {
+        const uint32_t d = (uint32_t)(c - j0);
+        const uint32_t vb = ((wsel<NW>(vis, j0 >> 5) >> (j0 & 31)) | (d < 4u ? 1u << d : 0u)) & 0xfu;
+        const uint32_t fb = (wsel<NW>(fnow, j0 >> 5) >> (j0 & 31)) & 0xfu;
+        const D4 T = ld4(tr, j0), R = ld4(rr, j0), D = ld4(dd, j0), S = ld4(F.score, j0),
+                 C = ld4(F.close, j0);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const int q = j0 + k;
             const bool open = !((vb >> k) & 1u);
             const bool after = fin_c <= T.v[k];
