@@ -65,27 +65,22 @@ __device__ __forceinline__ FusedTabs tabs_of(const FusedTabs &tabs, const DevIns
 
 __device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
 
-struct D8 { double v[8]; };
-__device__ __forceinline__ D8 ld8(cdp row, int j0) {
-    // one address for the chunk, so that the eight reads merge into one
-    // 64-byte load (offsets in 32 bits cannot be shown not to wrap)
+template <int CH> struct DN { double v[CH]; };
+template <int CH>
+__device__ __forceinline__ DN<CH> ldn(cdp row, int j0) {
+    // one address for the chunk, so that its reads merge into one scalar
+    // load (offsets in 32 bits cannot be shown not to wrap)
     const __attribute__((address_space(4))) double *p =
         (const __attribute__((address_space(4))) double *)(row.base + (row.off + 8u * (uint32_t)j0));
-    D8 r;
+    DN<CH> r;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) r.v[k] = p[k];
+    for (int k = 0; k < CH; ++k) r.v[k] = p[k];
     return r;
 }
-
-struct D4 { double v[4]; };
-__device__ __forceinline__ D4 ld4(cdp row, int j0) {
-    const __attribute__((address_space(4))) double *p =
-        (const __attribute__((address_space(4))) double *)(row.base + (row.off + 8u * (uint32_t)j0));
-    D4 r;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) r.v[k] = p[k];
-    return r;
-}
+typedef DN<8> D8;
+typedef DN<4> D4;
+__device__ __forceinline__ D8 ld8(cdp row, int j0) { return ldn<8>(row, j0); }
+__device__ __forceinline__ D4 ld4(cdp row, int j0) { return ldn<4>(row, j0); }
 
 // True when every active lane holds the same value of v.
 __device__ __forceinline__ bool wave_uniform(int v) {
@@ -144,13 +139,18 @@ __device__ __forceinline__ void scan_robfam(const FusedTabs &F, int n, int c, do
               rr = F.r + (size_t)(c + 1) * F.ldp;
     double s = 0.0, b = 0.0, bk = -1.0;
     int bi = -1, ki = -1;
-    for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8c<NW>(vis, j0, c);
-        const D8 T = ld8(tn, j0), R = ld8(rr, j0);
-        D8 TP;
-        if (RKR) TP = ld8(tp, j0);
+    // With REACH-K-ROB's table the scan reads three tables per neighbour; four
+    // neighbours per chunk then keep their scalar loads from spilling.
+    constexpr int CH = RKR ? 4 : 8;
+    for (int j0 = 0; j0 < n; j0 += CH) {
+        const uint32_t d = (uint32_t)(c - j0);
+        const uint32_t vb = ((wsel<NW>(vis, j0 >> 5) >> (j0 & 31)) | (d < (uint32_t)CH ? 1u << d : 0u)) &
+                            ((1u << CH) - 1u);
+        const DN<CH> T = ldn<CH>(tn, j0), R = ldn<CH>(rr, j0);
+        DN<CH> TP;
+        if (RKR) TP = ldn<CH>(tp, j0);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < CH; ++k) {
             const int j = j0 + k;
             const bool open = !((vb >> k) & 1u);
             const bool ok = open && frob <= T.v[k];
