@@ -88,18 +88,27 @@ __device__ __forceinline__ bool wave_uniform(int v) {
     return __builtin_amdgcn_ballot_w64(v != v0) == 0;
 }
 
-// The eight bits j0..j0+7 of a per-lane bit set (j0 a multiple of 8).
-template <int NW>
-__device__ __forceinline__ uint32_t bits8(const uint32_t (&a)[NW], int j0) {
-    return (wsel<NW>(a, j0 >> 5) >> (j0 & 31)) & 0xffu;
-}
+// A per-lane bit set read in ascending chunks of CH bits from bit 0 (CH
+// divides 32): the word holding the chunk is selected once per 32 customers
+// rather than once per chunk.
+template <int NW, int CH>
+struct BitRun {
+    const uint32_t (&a)[NW];
+    uint32_t w;
+    __device__ __forceinline__ explicit BitRun(const uint32_t (&s)[NW]) : a(s), w(0u) {}
+    __device__ __forceinline__ uint32_t next(int j0) {
+        if ((j0 & 31) == 0) w = wsel<NW>(a, j0 >> 5);
+        return (w >> (j0 & 31)) & ((1u << CH) - 1u);
+    }
+};
 
-// The same bits with candidate c added to the set: the scans skip c itself,
-// and one OR per chunk takes the place of a compare per neighbour.
-template <int NW>
-__device__ __forceinline__ uint32_t bits8c(const uint32_t (&a)[NW], int j0, int c) {
+// Candidate c's bit in the CH-bit chunk at j0 (0 outside it).  The scans add
+// it to the visited bits and so skip c itself: one OR per chunk takes the
+// place of a compare per neighbour.
+template <int CH>
+__device__ __forceinline__ uint32_t cbit(int c, int j0) {
     const uint32_t d = (uint32_t)(c - j0);
-    return bits8<NW>(a, j0) | (d < 8u ? 1u << d : 0u);
+    return d < (uint32_t)CH ? 1u << d : 0u;
 }
 
 // NS (engine/sim_terms.hpp:18-34), MAXN (:76-93) and the first leg of
@@ -110,8 +119,9 @@ __device__ __forceinline__ void scan_nsfam(const FusedTabs &F, int n, int c, dou
     const cdp tr = F.t0n + (size_t)(c + 1) * F.ldp, rr = F.r + (size_t)(c + 1) * F.ldp;
     double s = 0.0, m = 0.0, b = 0.0;
     int bi = -1;
+    BitRun<NW, 8> vr(vis);
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8c<NW>(vis, j0, c);
+        const uint32_t vb = vr.next(j0) | cbit<8>(c, j0);
         const D8 T = ld8(tr, j0), R = ld8(rr, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -142,10 +152,9 @@ __device__ __forceinline__ void scan_robfam(const FusedTabs &F, int n, int c, do
     // With REACH-K-ROB's table the scan reads three tables per neighbour; four
     // neighbours per chunk then keep their scalar loads from spilling.
     constexpr int CH = RKR ? 4 : 8;
+    BitRun<NW, CH> vr(vis);
     for (int j0 = 0; j0 < n; j0 += CH) {
-        const uint32_t d = (uint32_t)(c - j0);
-        const uint32_t vb = ((wsel<NW>(vis, j0 >> 5) >> (j0 & 31)) | (d < (uint32_t)CH ? 1u << d : 0u)) &
-                            ((1u << CH) - 1u);
+        const uint32_t vb = vr.next(j0) | cbit<CH>(c, j0);
         const DN<CH> T = ldn<CH>(tn, j0), R = ldn<CH>(rr, j0);
         DN<CH> TP;
         if (RKR) TP = ldn<CH>(tp, j0);
@@ -178,8 +187,9 @@ __device__ __forceinline__ double scan_regret(const FusedTabs &F, int n, int c, 
                               const uint32_t (&vis)[NW], const uint32_t (&fnow)[NW]) {
     const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp;
     double s = 0.0;
+    BitRun<NW, 8> fr(fnow), vr(vis);
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t mb = bits8<NW>(fnow, j0) & ~bits8c<NW>(vis, j0, c);
+        const uint32_t mb = fr.next(j0) & ~(vr.next(j0) | cbit<8>(c, j0));
         const D8 T = ld8(tr, j0), S = ld8(F.score, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -202,10 +212,10 @@ __device__ __forceinline__ void scan_struct(const FusedTabs &F, int n, int c, do
     int bi = -1;
     // Four neighbours per chunk: five streams of eight would hold 80 scalar
     // registers and spill inside the loop.
+    BitRun<NW, 4> vr(vis), fr(fnow);
     for (int j0 = 0; j0 < n; j0 += 4) {
-        const uint32_t d = (uint32_t)(c - j0);
-        const uint32_t vb = ((wsel<NW>(vis, j0 >> 5) >> (j0 & 31)) | (d < 4u ? 1u << d : 0u)) & 0xfu;
-        const uint32_t fb = (wsel<NW>(fnow, j0 >> 5) >> (j0 & 31)) & 0xfu;
+        const uint32_t vb = vr.next(j0) | cbit<4>(c, j0);
+        const uint32_t fb = fr.next(j0);
         const D4 T = ld4(tr, j0), R = ld4(rr, j0), D = ld4(dd, j0), S = ld4(F.score, j0),
                  C = ld4(F.close, j0);
 #pragma unroll
@@ -233,8 +243,9 @@ __device__ __forceinline__ int scan_reach1(const FusedTabs &F, int n, int c, dou
     const cdp tr = F.t0 + (size_t)(c + 1) * F.ldp, rr = F.r + (size_t)(c + 1) * F.ldp;
     double bv = -1.0;
     int bi = -1;
+    BitRun<NW, 8> vr(vis);
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8c<NW>(vis, j0, c);
+        const uint32_t vb = vr.next(j0) | cbit<8>(c, j0);
         const D8 T = ld8(tr, j0), R = ld8(rr, j0);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -255,8 +266,9 @@ __device__ __forceinline__ double scan_nstw(const FusedTabs &F, int n, int c, do
     const size_t row = (size_t)(c + 1) * F.ldp;
     const cdp tr = F.t0n + row, rr = F.r + row, dd = F.d + row;
     double s = 0.0;
+    BitRun<NW, 8> vr(vis);
     for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = bits8c<NW>(vis, j0, c);
+        const uint32_t vb = vr.next(j0) | cbit<8>(c, j0);
         const D8 T = ld8(tr, j0);
         uint32_t okm = 0;
 #pragma unroll
@@ -365,8 +377,9 @@ __device__ __forceinline__ double reach2_leg2(const DevInst &I, const FusedTabs 
         // every lane continues from the same neighbour: its row, read as scalars
         const int pu = __builtin_amdgcn_readfirstlane(pj), bu = pu - 1;
         const cdp tr = (rob ? F.t2n : F.t0n) + (size_t)pu * F.ldp, rr = F.r + (size_t)pu * F.ldp;
+        BitRun<NW, 8> vr(vis);
         for (int k0 = 0; k0 < I.n; k0 += 8) {
-            const uint32_t vb = bits8<NW>(vis, k0);
+            const uint32_t vb = vr.next(k0);
             const D8 T = ld8(tr, k0), R = ld8(rr, k0), S = ld8(F.score, k0);
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -380,8 +393,9 @@ __device__ __forceinline__ double reach2_leg2(const DevInst &I, const FusedTabs 
         return I.score[bu] + s2;
     }
     const cdp tcol = rob ? F.t2nc : F.t0nc;
+    BitRun<NW, 4> vr(vis);
     for (int k0 = 0; k0 < I.n; k0 += 4) {
-        const uint32_t vb = (wsel<NW>(vis, k0 >> 5) >> (k0 & 31)) & 0xfu;
+        const uint32_t vb = vr.next(k0);
         double T[4], R[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -426,8 +440,9 @@ __device__ __forceinline__ void reach_path(const DevInst &I, const FusedTabs &F,
             const cdp tr = (rob ? F.t2 : F.t0) + (size_t)cu * F.ldp, rr = F.r + (size_t)cu * F.ldp;
             best = -1;
             double bv = -1.0;
+            BitRun<NW, 8> sr(scr);
             for (int q0 = 0; q0 < I.n; q0 += 8) {
-                const uint32_t sb = bits8<NW>(scr, q0);
+                const uint32_t sb = sr.next(q0);
                 const D8 T = ld8(tr, q0), R = ld8(rr, q0);
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
@@ -440,8 +455,9 @@ __device__ __forceinline__ void reach_path(const DevInst &I, const FusedTabs &F,
         } else if (step > 0) {
             best = -1;
             double bv = -1.0;
+            BitRun<NW, 4> sr(scr);
             for (int q0 = 0; q0 < I.n; q0 += 4) {
-                const uint32_t sb = (wsel<NW>(scr, q0 >> 5) >> (q0 & 31)) & 0xfu;
+                const uint32_t sb = sr.next(q0);
                 double T[4], R[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
