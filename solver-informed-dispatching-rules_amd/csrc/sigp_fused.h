@@ -13,10 +13,10 @@
 // still the reference's own expression (engine/sim_terms.hpp).
 //
 // Each family is a short loop specialised at compile time for the members
-// the rule needs, reading eight neighbours per scalar-load chunk (four where
-// a scan reads three or more tables, so that the chunks fit the scalar
-// registers) from rows padded to a multiple of 8 (padding has tau = -inf, so
-// it never counts).
+// the rule needs, reading eight neighbours per scalar-load chunk (four in the
+// structural scan and in the robust scan with REACH-K-ROB's table, whose
+// table streams do not fit the scalar registers at eight) from rows padded to
+// a multiple of 8 (padding has tau = -inf, so it never counts).
 // Scans that start from a lane's own POI (second legs, later REACH-K
 // waypoints) gather down the columns of the transposed tables, whose rows are
 // padded the same way.
