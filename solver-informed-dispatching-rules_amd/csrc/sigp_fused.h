@@ -14,9 +14,9 @@
 //
 // Each family is a short loop specialised at compile time for the members
 // the rule needs, reading eight neighbours per scalar-load chunk (four in the
-// structural scan and in the robust scan with REACH-K-ROB's table, whose
-// table streams do not fit the scalar registers at eight) from rows padded to
-// a multiple of 8 (padding has tau = -inf, so it never counts).
+// structural and NSTW scans and in the robust scan with REACH-K-ROB's table,
+// whose table streams do not fit the scalar registers at eight) from rows
+// padded to a multiple of 8 (padding has tau = -inf, so it never counts).
 // Scans that start from a lane's own POI (second legs, later REACH-K
 // waypoints) gather down the columns of the transposed tables, whose rows are
 // padded the same way.
@@ -261,35 +261,37 @@ __device__ __forceinline__ int scan_reach1(const FusedTabs &F, int n, int c, dou
 }
 
 // NSTW (engine/sim_terms.hpp:192-213).  The weight is exactly 1 unless the
-// lane arrives before the window opens, so the common case is a plain
-// add of r (r * 1.0 == r); the division path runs one neighbour at a time.
+// lane arrives before the window opens (with no wait, 1 - 0 / win == 1), so
+// the division runs only for lanes that wait, and r * 1.0 == r is added for
+// the rest.  Four neighbours per chunk: the five tables a chunk with an
+// eligible neighbour reads fit the scalar registers.
 template <int NW>
 __device__ __forceinline__ double scan_nstw(const FusedTabs &F, int n, int c, double fin, const uint32_t (&vis)[NW]) {
     const size_t row = (size_t)(c + 1) * F.ldp;
     const cdp tr = F.t0n + row, rr = F.r + row, dd = F.d + row;
     double s = 0.0;
-    BitRun<NW, 8> vr(vis);
-    for (int j0 = 0; j0 < n; j0 += 8) {
-        const uint32_t vb = vr.next(j0) | cbit<8>(c, j0);
-        const D8 T = ld8(tr, j0);
+    BitRun<NW, 4> vr(vis);
+    for (int j0 = 0; j0 < n; j0 += 4) {
+        const uint32_t vb = vr.next(j0) | cbit<4>(c, j0);
+        const D4 T = ld4(tr, j0);
         uint32_t okm = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < 4; ++k) {
             const bool ok = !((vb >> k) & 1u) && fin <= T.v[k];
             okm |= (ok ? 1u : 0u) << k;
         }
         if (__builtin_amdgcn_ballot_w64(okm != 0) == 0) continue;
-#pragma unroll 1
-        for (int k = 0; k < 8; ++k) {
-            const int j = j0 + k;
+        const D4 R = ld4(rr, j0), O = ld4(F.open, j0), C = ld4(F.close, j0), D = ld4(dd, j0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
             if (!((okm >> k) & 1u)) continue;
-            const double r = rr[j], o = F.open[j], cl = F.close[j];
-            double wait = o - (fin + dd[j]);
+            double wait = O.v[k] - (fin + D.v[k]);
             if (wait < 0.0) wait = 0.0;
-            const double win = cl - o;
-            double w = win > 1e-10 ? 1.0 - wait / win : 1.0;
+            const double win = C.v[k] - O.v[k];
+            double w = 1.0;
+            if (wait > 0.0 && win > 1e-10) w = 1.0 - wait / win;
             if (w < 0.0) w = 0.0;
-            s += r * w;
+            s += R.v[k] * w;
         }
     }
     return s;
