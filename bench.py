@@ -34,7 +34,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 (vector and matrix) peak, FMA 
 # HBM bytes per simulation-kernel launch, keyed by (rules, scenarios): the
 # FETCH_SIZE and WRITE_SIZE counters of profiles/sim_kernel_plain_small_hbm_traffic.md,
 # each scaled by what it reads for a plain copy of known size (x1.9997, x1.0).
-TRAFFIC_BYTES = {(1024, 500): 21388401}
+TRAFFIC_BYTES = {(1024, 500): 21430034}
 MAPPINGS = {"scenarios": 1, "candidates": 2, "candidates2": 3}
 
 
